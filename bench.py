@@ -1,0 +1,271 @@
+#!/usr/bin/env python3
+"""Benchmark of the VoxelConstrainedSphere superposition path on one MI355X.
+
+Metric: deposits scored per second (one deposit = one point of a track in one
+oversample), for the batched FP32 form, with the plain FP64 drop-in path and
+the batched FP64 form measured beside it, and the FP64 CPU oracle timed on a
+bounded sample of the same workload.
+
+A step is one pass over the whole track library: every track x every
+oversample, from the shifts to the histogram counts copied back to the host.
+Inputs (the synthetic library) are resident in HBM before the timed region.
+
+    python bench.py                      # default: cfg4 batched FP32 (+ cfg3 comparisons)
+    python bench.py --workload cfg3 --steps 3 --warmup 1
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=3)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--workload", default="cfg4")
+    p.add_argument("--tracks", type=int, default=0, help="use only the first T tracks (0: all)")
+    p.add_argument("--oversamples", type=int, default=0, help="override the workload's oversamples")
+    p.add_argument("--no-compare", action="store_true", help="skip the plain-FP64 / batched-FP64 legs")
+    p.add_argument("--plain-stride", type=int, default=16, help="plain FP64 leg: every k-th track of cfg3")
+    p.add_argument("--cpu-stride", type=int, default=64, help="CPU oracle sample: every k-th track")
+    p.add_argument("--cpu-oversamples", type=int, default=0, help="CPU sample oversamples (0: workload's)")
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--autotune", action="store_true")
+    return p.parse_args()
+
+
+class Events:
+    """HIP events recorded on a given stream (through the HIP runtime)."""
+
+    def __init__(self):
+        self.hip = C.CDLL("libamdhip64.so")
+        self.hip.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
+        self.hip.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+        self.hip.hipEventSynchronize.argtypes = [C.c_void_p]
+        self.hip.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+
+    def create(self):
+        e = C.c_void_p()
+        assert self.hip.hipEventCreate(C.byref(e)) == 0
+        return e
+
+    def record(self, e, stream):
+        assert self.hip.hipEventRecord(e, stream) == 0
+
+    def elapsed_ms(self, a, b):
+        self.hip.hipEventSynchronize(b)
+        ms = C.c_float(0)
+        assert self.hip.hipEventElapsedTime(C.byref(ms), a, b) == 0
+        return float(ms.value)
+
+
+def build_library(w, n_tracks, precision):
+    import supertrack_amd as st
+    from supertrack_amd import workloads as W
+    t0 = time.time()
+    x, y, z, e, off, gt = W.make_library(w, n_tracks=n_tracks)
+    t1 = time.time()
+    geoms = W.geometries(w)
+    lib = st.TrackLibrary(x, y, z, e, off, geoms, gt if len(geoms) > 1 else None, precision)
+    t2 = time.time()
+    log(f"library: {n_tracks} tracks, {int(off[-1])} points; generate {t1 - t0:.1f}s, "
+        f"layout+upload {t2 - t1:.1f}s; {lib.info()}")
+    return lib, (x, y, z, e, off, gt)
+
+
+def time_batch(lib, w, n_os, steps, warmup, autotune, ev):
+    """Returns (seconds per step, kernel ms per step, params, counts)."""
+    import supertrack_amd as st
+    from supertrack_amd import workloads as W
+    hs = [st.Histogram(nb, lo, hi, kind) for kind, nb, lo, hi in w["hists"]]
+    b = st.BatchRun(lib, hs, n_os, W.SEED, True, True, use_graph=True, autotune=autotune)
+    stream = st.default_stream_handle()
+    for _ in range(warmup):
+        b.run()
+        b.sync()
+        for h in hs:
+            h.fetch(reset=True)
+    st.synchronize()
+    e0, e1 = ev.create(), ev.create()
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ev.record(e0, stream)
+        b.run()
+        ev.record(e1, stream)
+        b.sync()
+        counts = [h.fetch(reset=True) for h in hs]   # bins back on the host: part of a step
+        kernel_ms.append(ev.elapsed_ms(e0, e1))
+    st.synchronize()
+    secs = (time.perf_counter() - t0) / steps
+    params = b.params()
+    b.free()
+    for h in hs:
+        h.free()
+    return secs, float(np.mean(kernel_ms)), params, counts
+
+
+def time_plain(w, host, stride, n_os):
+    """Plain FP64 drop-in path (per track: begin, n_os x (process, sort-reduce-bin), end)."""
+    import supertrack_amd as st
+    from supertrack_amd import workloads as W
+    x, y, z, e, off, gt = host
+    tracks = list(range(0, len(off) - 1, stride))
+    geoms = W.geometries(w)
+    kind, nb, lo, hi = w["hists"][0]
+    h = st.Histogram(nb, lo, hi, kind)
+    methods = [st.VoxelConstrainedSphereMethod(g, True, True) for g in geoms]
+    # tracks are loaded to the device outside the timed region (ingest is not part of the path)
+    dev = []
+    cap = 0
+    for t in tracks:
+        b, en = int(off[t]), int(off[t + 1])
+        dev.append(st.DeviceArray.from_numpy(np.stack([x[b:en], y[b:en], z[b:en], e[b:en]])))
+        cap = max(cap, en - b)
+    trk = st.Track(capacity=cap)
+    pairs = st.VolumeEdepPair(cap)
+    st.synchronize()
+    deposits = 0
+    t0 = time.perf_counter()
+    for j, t in enumerate(tracks):
+        n = int(off[t + 1] - off[t])
+        base = dev[j].ptr.value
+        trk.copy_from_device([base, base + 8 * n, base + 16 * n, base + 24 * n], n)
+        m = methods[int(gt[t]) if len(geoms) > 1 else 0]
+        pairs.c.capacity = n
+        m.allocate_track_process(trk, W.SEED, t, n_os)
+        h.allocate_track_process(pairs)
+        for _ in range(n_os):
+            m.process_track(trk, pairs)
+            h.sort_reduce_and_add_to_histogram(pairs)
+        m.free_track_process()
+        h.free_track_process()
+        deposits += n * n_os
+    st.synchronize()
+    secs = time.perf_counter() - t0
+    counts = h.counts()
+    pairs.c.capacity = cap
+    return deposits, secs, len(tracks), counts
+
+
+def cpu_baseline(w, stride, n_os):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    from supertrack_amd import workloads as W
+    n_sample = (w["n_tracks"] + stride - 1) // stride
+    x, y, z, e, off, gt = W.make_library(w, n_tracks=n_sample, t0=0, stride=stride)
+    geoms = [po.geometry(h, d) for h, d in w["geometries"]]
+    edges = [po.edges(kind, lo, hi, nb) for kind, nb, lo, hi in w["hists"]]
+    # tracks of the sample keep their library index for the shift stream
+    deposits = int(off[-1]) * n_os
+    total = 0.0
+    threads = 0
+    for j in range(n_sample):
+        sub = slice(int(off[j]), int(off[j + 1]))
+        o1 = np.array([0, int(off[j + 1] - off[j])], np.int64)
+        _, secs, threads = po.run(x[sub], y[sub], z[sub], e[sub], o1, geoms, gt[j:j + 1], n_os, W.SEED,
+                                  True, True, edges, track_index_base=j * stride, n_threads=0)
+        total += secs
+    return deposits / total, threads, n_sample, deposits, total
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    import supertrack_amd as st
+    from supertrack_amd import workloads as W
+
+    ev = Events()
+    w = W.workload(args.workload)
+    n_tracks = args.tracks or w["n_tracks"]
+    n_os = args.oversamples or w["n_oversamples"]
+    lib, host = build_library(w, n_tracks, w["precision"])
+    info = lib.info()
+    total_points = info["n_points"]
+    deposits = total_points * n_os
+    secs, kms, params, counts = time_batch(lib, w, n_os, args.steps, args.warmup, args.autotune, ev)
+    value = deposits / secs
+    bytes_per_dep = 16 if w["precision"] == st.ST_F32 else 32
+    achieved = deposits * bytes_per_dep / (kms * 1e-3) / 1e9
+    log(f"{args.workload} batched: {secs * 1e3:.1f} ms/step, kernel {kms:.1f} ms, {value:.3e} deposits/s, "
+        f"params {params}")
+    lib.free()
+    del host
+
+    extra = {}
+    if not args.no_compare:
+        w3 = W.workload("cfg3")
+        lib3, host3 = build_library(w3, args.tracks or w3["n_tracks"], st.ST_F64)
+        d3 = lib3.info()["n_points"] * w3["n_oversamples"]
+        s64, k64, _, _ = time_batch(lib3, w3, w3["n_oversamples"], 1, 1, False, ev)
+        lib3.free()
+        lib3f = st.TrackLibrary(*host3[:5], W.geometries(w3), None, st.ST_F32)
+        s32, k32, _, _ = time_batch(lib3f, w3, w3["n_oversamples"], 1, 1, False, ev)
+        lib3f.free()
+        pd, ps, pn, _ = time_plain(w3, host3, args.plain_stride, w3["n_oversamples"])
+        plain_rate = pd / ps
+        extra = {
+            "cfg3_batched_fp64_deposits_per_s": d3 / s64,
+            "cfg3_batched_fp32_deposits_per_s": d3 / s32,
+            "cfg3_plain_fp64_deposits_per_s": plain_rate,
+            "cfg3_plain_fp64_sample": f"every {args.plain_stride}th track ({pn} tracks) x {w3['n_oversamples']} oversamples",
+            "cfg3_speedup_batched_fp32_vs_plain_fp64": (d3 / s32) / plain_rate,
+        }
+        log(f"cfg3: batched fp64 {d3 / s64:.3e}/s, batched fp32 {d3 / s32:.3e}/s, plain fp64 {plain_rate:.3e}/s")
+
+    cpu = None
+    if not args.no_cpu:
+        cos = args.cpu_oversamples or n_os
+        rate, threads, ns, cdep, csecs = cpu_baseline(w, args.cpu_stride, cos)
+        cpu = {"value": rate, "unit": "deposits/s", "cores": threads, "kind": "port",
+               "sample": f"{args.workload}: every {args.cpu_stride}th track ({ns} tracks) x {cos} oversamples, "
+                         f"{cdep:.3e} deposits in {csecs:.1f}s, FP64 oracle, OpenMP"}
+        log(f"cpu oracle: {rate:.3e} deposits/s on {threads} threads")
+
+    out = {
+        "metric": "deposits scored/s on 1 MI355X, batched-FP32 vs plain-FP64; % of HBM peak",
+        "value": value,
+        "unit": "deposits/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": secs * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if w["precision"] == st.ST_F32 else "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{args.workload}: {n_tracks} tracks, {total_points} points, {n_os} oversamples, "
+                               f"geometries {w['geometries']}, histograms {w['hists']}",
+                   "deposits_per_step": deposits, "parallelism": "1 GPU", "launch": params},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "basis": f"{bytes_per_dep} B per deposit x {deposits} deposits / batch kernel time"},
+        "cpu_baseline": cpu,
+    }
+    out.update(extra)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

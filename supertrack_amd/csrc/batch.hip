@@ -1,0 +1,1121 @@
+// batch.hip -- the batched form of the VoxelConstrainedSphere path: a resident
+// track library and one persistent fused kernel that scores every
+// (track, oversample) segment: shift -> voxel -> sphere -> key -> per-sphere
+// sum -> lineal energy -> bin.
+//
+// Library layout (HBM), built once by st_library_create:
+//   * each track's points sorted by z (a rigid shift keeps that order, so for
+//     any oversample the points inside the voxel in z form one contiguous run
+//     and the sphere layer index iz never decreases along the run);
+//   * FP64 SoA x, y, z, edep in that order (the data in ST_F64 mode, the exact
+//     side copy in ST_F32 mode);
+//   * ST_F32 mode: FP32 SoA dx, dy, dz, edep, with coordinates relative to a
+//     per-256-point block origin (FP64) and a per-block bound on the FP32
+//     rounding error;
+//   * per block: z of its first point (the search index); per track: offsets,
+//     x/y bounds, geometry, eligibility.
+//
+// Kernel: each workgroup takes segments from a global ticket (largest tracks
+// first, all oversamples of a track together so the track stays in L2/MALL).
+// Per segment it culls by the track's x/y bounds, finds the in-voxel block
+// range with a 64-way search over block starts, and streams the range in
+// chunks of 256 points.  In-sphere points are accumulated per sphere key in an
+// LDS open-addressing table as fixed-point integers (exact and independent of
+// order, so results are bit-reproducible and do not depend on launch
+// parameters).  When the table fills past half, every entry whose key lies
+// below the first layer still to come is complete: those are converted to
+// lineal energy and binned into LDS-privatised histograms; the rest are kept.
+// Tracks whose densest 4-layer window could hold more spheres than the table
+// keeps (or with non-finite data) are scored by the exact per-segment path.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <new>
+#include <numeric>
+#include <vector>
+
+#include "st_common.hpp"
+#include "st_internal.hpp"
+
+namespace st {
+namespace {
+
+constexpr int kBT = 256;                    // threads per workgroup
+constexpr int kBlk = 256;                   // points per library block (== chunk)
+constexpr unsigned long long kEmpty = ~0ull;
+constexpr int kMaxGeoms = 16;
+constexpr int kMaxHists = 2;
+constexpr int64_t kMaxWindow = 512;         // fast-path bound on spheres in a 4-layer window
+constexpr int64_t kMaxFastPoints = (int64_t)1 << 22;
+constexpr uint32_t kFlagExact = 1u;
+
+constexpr double kTwo24 = 16777216.0;
+constexpr double kTwo40 = 1099511627776.0;
+constexpr double kTwo32 = 4294967296.0;
+constexpr double kTwoM24 = 1.0 / 16777216.0;
+constexpr double kTwoM64 = 1.0 / 18446744073709551616.0;
+constexpr double kTwoM32 = 1.0 / 4294967296.0;
+constexpr double kTwoM44 = 1.0 / 17592186044416.0;
+
+struct TrackDesc {
+    int64_t offset;     // first point in the sorted arrays
+    int64_t block0;     // first block in the block arrays
+    int32_t n;
+    int32_t nblocks;
+    int32_t geom;
+    uint32_t flags;
+    double xmin, xmax, ymin, ymax, zmax;
+};
+
+struct GeomDev {
+    double H, L, D, invD, r2, gso, denom, denom_inv;
+    unsigned long long N, N2;
+};
+
+struct HistDev {
+    const double *edges;
+    unsigned long long *bins;
+    BinGuess guess;
+    int lds_offset;     // in u32 units from the histogram LDS base
+};
+
+struct BatchArgs {
+    // library
+    const TrackDesc *tracks;
+    const double *x64, *y64, *z64, *e64;
+    const float *dx32, *dy32, *dz32, *e32;
+    const double *zfirst;
+    const double *origin;   // 3 per block
+    const double *eps;      // per block
+    const GeomDev *geoms;
+    // work
+    const int32_t *order;
+    int64_t n_seg;
+    int32_t n_os;
+    int32_t shift_xy, shift_z;
+    uint32_t track_index_base;
+    uint64_t seed;
+    unsigned long long *ticket;
+    int *error;
+    // histograms
+    HistDev hist[kMaxHists];
+    int n_hists;
+    int hist_lds_words;
+    // recording (test hook)
+    int64_t rec_cap;
+    unsigned long long *rec_count;
+    int64_t *rec_seg;
+    unsigned long long *rec_key;
+    double *rec_sum;
+};
+
+// ---------------------------------------------------------------------------
+// exact FP64 membership (the reference's operations, rounding as written)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double nearest_index(double a, double D, double invD)
+{
+    const double q = a * invD;
+    double k = rint(q);
+    const double f = fabs(q - k);
+    if (f > 0.5 - (fabs(q) * 1e-15 + 1e-300)) k = rint(a / D);
+    return k;
+}
+
+__device__ __forceinline__ bool exact_member(double x, double y, double zp, const GeomDev &g,
+                                             double sx, double sy, double sz, unsigned long long &key)
+{
+    const double xs = x + sx;
+    const double ys = y + sy;
+    const double zs = zp + sz;
+    if (!(fabs(xs) < g.H && fabs(ys) < g.H && fabs(zs) < g.H)) return false;
+    const double ax = xs - g.gso, ay = ys - g.gso, az = zs - g.gso;
+    const double kx = nearest_index(ax, g.D, g.invD);
+    const double ky = nearest_index(ay, g.D, g.invD);
+    const double kz = nearest_index(az, g.D, g.invD);
+    const double dx = (kx * g.D) - ax;
+    const double dy = (ky * g.D) - ay;
+    const double dz = (kz * g.D) - az;
+    const double dist = ((dx * dx) + (dy * dy)) + (dz * dz);
+    if (!(dist <= g.r2)) return false;
+    key = (unsigned long long)(long long)kx + (unsigned long long)(long long)ky * g.N +
+          (unsigned long long)(long long)kz * g.N * g.N;
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// fixed-point accumulation
+// ---------------------------------------------------------------------------
+// ST_F64: y = hi * 2^-24 + lo * 2^-64 with hi = floor(y * 2^24); both parts
+// exact (lo rounded at 2^-65), sums exact while they stay in int64.
+// ST_F32: y = acc * 2^-32.
+template <int MODE>
+struct Acc;
+
+template <>
+struct Acc<ST_F64> {
+    static constexpr int words = 2;
+    __device__ static void split(double y, long long &a0, long long &a1)
+    {
+        const double t = y * kTwo24;
+        const double fh = floor(t);
+        a0 = (long long)fh;
+        a1 = (long long)rint((t - fh) * kTwo40);
+    }
+    __device__ static double value(unsigned long long a0, unsigned long long a1)
+    {
+        return (double)(long long)a0 * kTwoM24 + (double)(long long)a1 * kTwoM64;
+    }
+};
+
+template <>
+struct Acc<ST_F32> {
+    static constexpr int words = 1;
+    __device__ static void split(double y, long long &a0, long long &a1)
+    {
+        a0 = (long long)rint(y * kTwo32);
+        a1 = 0;
+    }
+    __device__ static double value(unsigned long long a0, unsigned long long)
+    {
+        return (double)(long long)a0 * kTwoM32;
+    }
+};
+
+__device__ __forceinline__ unsigned int slot_hash(unsigned long long key, int log2_slots)
+{
+    return (unsigned int)((key * 0x9E3779B97F4A7C15ull) >> (64 - log2_slots));
+}
+
+// ---------------------------------------------------------------------------
+// 64-way lower bound by one wave: first i in [0, n) with pred(i), else n.
+// ---------------------------------------------------------------------------
+template <typename Pred>
+__device__ int wave_lower_bound(int n, Pred pred)
+{
+    const int lane = threadIdx.x & 63;
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int len = hi - lo;
+        const int step = (len + 63) / 64;
+        const int i = lo + lane * step;
+        const bool p = (i < hi) ? pred(i) : true;
+        const unsigned long long mask = __ballot(p);
+        if (mask == 0ull) {
+            lo = lo + 63 * step + 1;
+        } else {
+            const int f = __ffsll((long long)mask) - 1;
+            const int nhi = lo + f * step;
+            const int nlo = (f == 0) ? lo : lo + (f - 1) * step + 1;
+            lo = nlo;
+            hi = min(nhi, hi);
+        }
+    }
+    return lo;
+}
+
+// ---------------------------------------------------------------------------
+// the persistent fused kernel
+// ---------------------------------------------------------------------------
+template <int MODE, int SLOTS>
+__global__ __launch_bounds__(kBT) void batch_kernel(BatchArgs a)
+{
+    constexpr int kLog2Slots = (SLOTS == 1024) ? 10 : (SLOTS == 2048) ? 11 : 12;
+    constexpr int kPer = SLOTS / kBT;            // table slots scanned per thread
+    constexpr int kWords = Acc<MODE>::words;
+    extern __shared__ unsigned long long lds[];
+    unsigned long long *tkey = lds;
+    unsigned long long *tacc0 = tkey + SLOTS;
+    unsigned long long *tacc1 = tacc0 + SLOTS;  // used when kWords == 2
+    unsigned int *lhist = reinterpret_cast<unsigned int *>(tacc0 + SLOTS * kWords);
+
+    __shared__ long long s_seg;
+    __shared__ int s_occ;
+    __shared__ int s_range[2];
+    __shared__ unsigned long long s_binned;
+
+    const int tid = threadIdx.x;
+    for (int s = tid; s < SLOTS; s += kBT) {
+        tkey[s] = kEmpty;
+        tacc0[s] = 0;
+        if (kWords == 2) tacc1[s] = 0;
+    }
+    for (int w = tid; w < a.hist_lds_words; w += kBT) lhist[w] = 0;
+    if (tid == 0) {
+        s_occ = 0;
+        s_binned = 0;
+    }
+    __syncthreads();
+
+    // bins one complete sphere sum into every histogram
+    auto emit = [&](unsigned long long key, unsigned long long a0, unsigned long long a1, int64_t seg) {
+        const double v = Acc<MODE>::value(a0, a1);
+#pragma unroll
+        for (int h = 0; h < kMaxHists; ++h) {
+            if (h < a.n_hists) {
+                const HistDev &hd = a.hist[h];
+                const int b = bin_of(hd.edges, hd.guess.nbins, v, bin_guess(hd.guess, v));
+                if (b >= 0) atomicAdd(&lhist[hd.lds_offset + b], 1u);
+            }
+        }
+        if (a.rec_cap > 0) {
+            const unsigned long long p = atomicAdd(a.rec_count, 1ull);
+            if ((int64_t)p < a.rec_cap) {
+                a.rec_seg[p] = seg;
+                a.rec_key[p] = key;
+                a.rec_sum[p] = v;
+            }
+        }
+    };
+
+    // flushes entries with key < bound (all entries when bound == kEmpty)
+    auto flush = [&](unsigned long long bound, int64_t seg) {
+        __syncthreads();
+        unsigned long long kk[kPer], k0[kPer], k1[kPer];
+        int nbin = 0;
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            const int s = tid + j * kBT;
+            const unsigned long long k = tkey[s];
+            kk[j] = kEmpty;
+            if (k != kEmpty) {
+                const unsigned long long v0 = tacc0[s];
+                const unsigned long long v1 = (kWords == 2) ? tacc1[s] : 0ull;
+                if (bound == kEmpty || k < bound) {
+                    emit(k, v0, v1, seg);
+                    ++nbin;
+                } else {
+                    kk[j] = k;
+                    k0[j] = v0;
+                    k1[j] = v1;
+                }
+                tkey[s] = kEmpty;
+                tacc0[s] = 0;
+                if (kWords == 2) tacc1[s] = 0;
+            }
+        }
+        if (nbin) atomicAdd(&s_binned, (unsigned long long)nbin);
+        __syncthreads();
+        if (tid == 0) s_occ = 0;
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            if (kk[j] != kEmpty) {
+                unsigned int h = slot_hash(kk[j], kLog2Slots);
+                for (int probe = 0; probe < SLOTS; ++probe) {
+                    const unsigned long long prev = atomicCAS(&tkey[h], kEmpty, kk[j]);
+                    if (prev == kEmpty) {
+                        atomicAdd(&s_occ, 1);
+                        break;
+                    }
+                    h = (h + 1) & (SLOTS - 1);
+                }
+                atomicAdd(&tacc0[h], k0[j]);
+                if (kWords == 2) atomicAdd(&tacc1[h], k1[j]);
+            }
+        }
+        __syncthreads();
+    };
+
+    for (;;) {
+        if (tid == 0) s_seg = (long long)atomicAdd(a.ticket, 1ull);
+        __syncthreads();
+        const int64_t seg = s_seg;
+        __syncthreads();
+        if (seg >= a.n_seg) break;
+        const int32_t t = a.order[seg / a.n_os];
+        const int32_t os = (int32_t)(seg % a.n_os);
+        const TrackDesc td = a.tracks[t];
+        const GeomDev g = a.geoms[td.geom];
+        const Shift sh = shift_for(a.seed, a.track_index_base + (uint32_t)t, (uint32_t)os, a.shift_xy,
+                                   a.shift_z, g.L, g.H);
+        const int64_t seg_id = (int64_t)t * a.n_os + os;
+        // x/y bounds cull: fl(x + s) is monotone in x
+        if (!((td.xmax + sh.x) > -g.H && (td.xmin + sh.x) < g.H && (td.ymax + sh.y) > -g.H &&
+              (td.ymin + sh.y) < g.H))
+            continue;
+        // in-voxel block range: blocks whose z span can reach inside (-H, H)
+        if (tid < 64) {
+            const double *zf = a.zfirst + td.block0;
+            const int nb = td.nblocks;
+            auto zs_of = [&](double z) { return (z + g.H) + sh.z; };
+            const int b_lo = wave_lower_bound(nb, [&](int b) {
+                const double zhi = (b + 1 < nb) ? zf[b + 1] : td.zmax;
+                return zs_of(zhi) > -g.H;
+            });
+            const int b_hi = wave_lower_bound(nb, [&](int b) { return !(zs_of(zf[b]) < g.H); });
+            if (tid == 0) {
+                s_range[0] = b_lo;
+                s_range[1] = b_hi;
+            }
+        }
+        __syncthreads();
+        const int b_lo = s_range[0], b_hi = s_range[1];
+        if (b_lo >= b_hi) continue;
+        const int64_t p_end = td.offset + min((int64_t)b_hi * kBlk, (int64_t)td.n);
+        for (int b = b_lo; b < b_hi; ++b) {
+            const int64_t c = td.offset + (int64_t)b * kBlk;
+            const int64_t i = c + tid;
+            bool in = false;
+            unsigned long long key = 0;
+            double yv = 0.0;
+            if (i < p_end) {
+                if (MODE == ST_F64) {
+                    in = exact_member(a.x64[i], a.y64[i], a.z64[i] + g.H, g, sh.x, sh.y, sh.z, key);
+                    if (in) yv = a.e64[i] / g.denom;
+                } else {
+                    const int64_t blk = td.block0 + b;
+                    const double ox = a.origin[3 * blk + 0];
+                    const double oy = a.origin[3 * blk + 1];
+                    const double oz = a.origin[3 * blk + 2];
+                    const double dl = a.eps[blk] + kTwoM44 * (fabs(ox) + fabs(oy) + fabs(oz) + fabs(sh.x) +
+                                                               fabs(sh.y) + fabs(sh.z) + 4.0 * g.H +
+                                                               fabs(g.gso) + g.D + 16.0);
+                    const double xs = (ox + sh.x) + (double)a.dx32[i];
+                    const double ys = (oy + sh.y) + (double)a.dy32[i];
+                    const double zs = ((oz + g.H) + sh.z) + (double)a.dz32[i];
+                    bool amb = fabs(fabs(xs) - g.H) <= dl || fabs(fabs(ys) - g.H) <= dl ||
+                               fabs(fabs(zs) - g.H) <= dl;
+                    if (!amb && fabs(xs) < g.H && fabs(ys) < g.H && fabs(zs) < g.H) {
+                        const double ax = xs - g.gso, ay = ys - g.gso, az = zs - g.gso;
+                        const double qx = ax * g.invD, qy = ay * g.invD, qz = az * g.invD;
+                        const double kx = rint(qx), ky = rint(qy), kz = rint(qz);
+                        const double fm = 0.5 - (dl * g.invD + 1e-12);
+                        amb = fabs(qx - kx) >= fm || fabs(qy - ky) >= fm || fabs(qz - kz) >= fm;
+                        if (!amb) {
+                            const double dx = kx * g.D - ax, dy = ky * g.D - ay, dz = kz * g.D - az;
+                            const double dist = dx * dx + dy * dy + dz * dz;
+                            const double mrg = 3.0 * dl * (g.D + 2.0 * dl) + 1e-14 * g.r2;
+                            amb = fabs(dist - g.r2) <= mrg;
+                            if (!amb && dist < g.r2) {
+                                in = true;
+                                key = (unsigned long long)(long long)kx +
+                                      (unsigned long long)(long long)ky * g.N +
+                                      (unsigned long long)(long long)kz * g.N * g.N;
+                            }
+                        }
+                    }
+                    if (amb) in = exact_member(a.x64[i], a.y64[i], a.z64[i] + g.H, g, sh.x, sh.y, sh.z, key);
+                    if (in) yv = (double)a.e32[i] * g.denom_inv;
+                }
+            }
+            if (in) {
+                long long v0, v1;
+                Acc<MODE>::split(yv, v0, v1);
+                unsigned int h = slot_hash(key, kLog2Slots);
+                int probe = 0;
+                for (; probe < SLOTS; ++probe) {
+                    const unsigned long long prev = atomicCAS(&tkey[h], kEmpty, key);
+                    if (prev == kEmpty) {
+                        atomicAdd(&s_occ, 1);
+                        break;
+                    }
+                    if (prev == key) break;
+                    h = (h + 1) & (SLOTS - 1);
+                }
+                if (probe == SLOTS) {
+                    atomicOr(a.error, 1);
+                } else {
+                    atomicAdd(&tacc0[h], (unsigned long long)v0);
+                    if (kWords == 2) atomicAdd(&tacc1[h], (unsigned long long)v1);
+                }
+            }
+            __syncthreads();
+            const int64_t next = c + kBlk;
+            if (s_occ >= SLOTS / 2 && next < p_end) {
+                // every later point lies in layer >= rint(q) - 1 of the next point
+                const double zs = (a.z64[next] + g.H) + sh.z;
+                const double q = (zs - g.gso) * g.invD;
+                double lb = rint(q) - 1.0;
+                if (!(lb > 0.0)) lb = 0.0;
+                const unsigned long long bound = (unsigned long long)lb * g.N2;
+                flush(bound, seg_id);
+            }
+        }
+        flush(kEmpty, seg_id);
+        if (s_binned > 0x40000000ull) {
+            // keep the 32-bit LDS counters far from overflow
+#pragma unroll
+            for (int h = 0; h < kMaxHists; ++h) {
+                if (h >= a.n_hists) continue;
+                const int nb = a.hist[h].guess.nbins;
+                for (int b = tid; b < nb; b += kBT) {
+                    const unsigned int v = lhist[a.hist[h].lds_offset + b];
+                    if (v) {
+                        atomicAdd(&a.hist[h].bins[b], (unsigned long long)v);
+                        lhist[a.hist[h].lds_offset + b] = 0;
+                    }
+                }
+            }
+            __syncthreads();
+            if (tid == 0) s_binned = 0;
+            __syncthreads();
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int h = 0; h < kMaxHists; ++h) {
+        if (h >= a.n_hists) continue;
+        const int nb = a.hist[h].guess.nbins;
+        for (int b = tid; b < nb; b += kBT) {
+            const unsigned int v = lhist[a.hist[h].lds_offset + b];
+            if (v) atomicAdd(&a.hist[h].bins[b], (unsigned long long)v);
+        }
+    }
+}
+
+__global__ void batch_reset_kernel(unsigned long long *ticket, int *error, unsigned long long *rec_count)
+{
+    if (threadIdx.x == 0) {
+        *ticket = 0;
+        *error = 0;
+        if (rec_count) *rec_count = 0;
+    }
+}
+
+template <int MODE, int SLOTS>
+const void *kernel_ptr()
+{
+    return (const void *)batch_kernel<MODE, SLOTS>;
+}
+
+const void *pick_kernel(int mode, int slots)
+{
+    if (mode == ST_F64) {
+        if (slots == 1024) return kernel_ptr<ST_F64, 1024>();
+        if (slots == 2048) return kernel_ptr<ST_F64, 2048>();
+        return kernel_ptr<ST_F64, 4096>();
+    }
+    if (slots == 1024) return kernel_ptr<ST_F32, 1024>();
+    if (slots == 2048) return kernel_ptr<ST_F32, 2048>();
+    return kernel_ptr<ST_F32, 4096>();
+}
+
+size_t lds_bytes(int mode, int slots, int hist_words)
+{
+    const int words = (mode == ST_F64) ? 2 : 1;
+    return (size_t)slots * 8 * (1 + words) + (size_t)hist_words * 4;
+}
+
+template <typename T>
+int dev_upload(T **dst, const T *src, size_t n, int64_t *bytes)
+{
+    *dst = nullptr;
+    if (n == 0) return ST_OK;
+    hipError_t e = hipMalloc((void **)dst, n * sizeof(T));
+    if (e != hipSuccess) {
+        *dst = nullptr;
+        return set_error(ST_ENOMEM, "st_library_create: hipMalloc(%zu B) failed: %s", n * sizeof(T),
+                         hipGetErrorString(e));
+    }
+    *bytes += (int64_t)(n * sizeof(T));
+    ST_HIP_TRY(hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice));
+    return ST_OK;
+}
+
+// Largest number of distinct spheres that the points of any z-window of
+// width w can touch, for any shift: min(points, x-cells * y-cells * layers).
+int64_t window_bound(const double *x, const double *y, const double *z, int64_t n, double D, double w,
+                     int64_t layers)
+{
+    if (n == 0) return 0;
+    std::deque<int64_t> xmax, xmin, ymax, ymin;
+    int64_t best = 0;
+    int64_t j = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        while (j < n && z[j] - z[i] <= w) {
+            while (!xmax.empty() && x[xmax.back()] <= x[j]) xmax.pop_back();
+            xmax.push_back(j);
+            while (!xmin.empty() && x[xmin.back()] >= x[j]) xmin.pop_back();
+            xmin.push_back(j);
+            while (!ymax.empty() && y[ymax.back()] <= y[j]) ymax.pop_back();
+            ymax.push_back(j);
+            while (!ymin.empty() && y[ymin.back()] >= y[j]) ymin.pop_back();
+            ymin.push_back(j);
+            ++j;
+        }
+        const int64_t cnt = j - i;
+        const double cx = std::floor((x[xmax.front()] - x[xmin.front()]) / D) + 2.0;
+        const double cy = std::floor((y[ymax.front()] - y[ymin.front()]) / D) + 2.0;
+        const double cells = cx * cy * (double)layers;
+        const int64_t b = cells < (double)cnt ? (int64_t)cells : cnt;
+        best = std::max(best, b);
+        if (best >= kMaxWindow + 1) return best;   // already ineligible
+        for (auto *dq : {&xmax, &xmin, &ymax, &ymin})
+            while (!dq->empty() && dq->front() <= i) dq->pop_front();
+    }
+    return best;
+}
+
+}  // namespace
+}  // namespace st
+
+using namespace st;
+
+struct st_library {
+    int64_t n_tracks = 0, n_points = 0, n_blocks = 0;
+    int precision = ST_F64;
+    std::vector<st_geometry> geoms;
+    std::vector<TrackDesc> tracks;
+    std::vector<int32_t> fast_order;   // fast tracks, largest first
+    std::vector<int32_t> exact;        // tracks for the exact path
+    int64_t device_bytes = 0;
+    double *x64 = nullptr, *y64 = nullptr, *z64 = nullptr, *e64 = nullptr;
+    float *dx32 = nullptr, *dy32 = nullptr, *dz32 = nullptr, *e32 = nullptr;
+    double *zfirst = nullptr, *origin = nullptr, *eps = nullptr;
+    TrackDesc *tracks_dev = nullptr;
+    GeomDev *geoms_dev = nullptr;
+};
+
+struct st_batch {
+    st_library *lib = nullptr;
+    st_batch_opts opts{};
+    std::vector<st_hist *> hists;
+    hipStream_t stream = nullptr;
+    int threads = kBT, slots = 2048, blocks_per_cu = 4, grid = 0;
+    int hist_words = 0;
+    size_t lds = 0;
+    int32_t *order_dev = nullptr;
+    unsigned long long *ticket = nullptr;
+    int *error = nullptr;
+    unsigned long long *rec_count = nullptr;
+    int64_t rec_cap = 0;
+    int64_t *rec_seg = nullptr;
+    unsigned long long *rec_key = nullptr;
+    double *rec_sum = nullptr;
+    bool graph_ready = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    // exact path
+    std::vector<st_vcs *> methods;    // one per geometry
+    st_track scratch{};
+    st_pairs pairs{};
+};
+
+namespace {
+
+BatchArgs make_args(st_batch *b)
+{
+    st_library *L = b->lib;
+    BatchArgs a{};
+    a.tracks = L->tracks_dev;
+    a.x64 = L->x64;
+    a.y64 = L->y64;
+    a.z64 = L->z64;
+    a.e64 = L->e64;
+    a.dx32 = L->dx32;
+    a.dy32 = L->dy32;
+    a.dz32 = L->dz32;
+    a.e32 = L->e32;
+    a.zfirst = L->zfirst;
+    a.origin = L->origin;
+    a.eps = L->eps;
+    a.geoms = L->geoms_dev;
+    a.order = b->order_dev;
+    a.n_seg = (int64_t)L->fast_order.size() * b->opts.n_oversamples;
+    a.n_os = std::max(b->opts.n_oversamples, 1);
+    a.shift_xy = b->opts.shift_xy;
+    a.shift_z = b->opts.shift_z;
+    a.track_index_base = b->opts.track_index_base;
+    a.seed = b->opts.seed;
+    a.ticket = b->ticket;
+    a.error = b->error;
+    a.n_hists = (int)b->hists.size();
+    int off = 0;
+    for (int h = 0; h < a.n_hists; ++h) {
+        a.hist[h].edges = b->hists[h]->edges_dev;
+        a.hist[h].bins = (unsigned long long *)b->hists[h]->bins_dev;
+        a.hist[h].guess = b->hists[h]->guess;
+        a.hist[h].lds_offset = off;
+        off += b->hists[h]->nbins;
+    }
+    a.hist_lds_words = off;
+    a.rec_cap = b->rec_cap;
+    a.rec_count = b->rec_count;
+    a.rec_seg = b->rec_seg;
+    a.rec_key = b->rec_key;
+    a.rec_sum = b->rec_sum;
+    return a;
+}
+
+int launch_fast(st_batch *b, int slots, int grid)
+{
+    BatchArgs a = make_args(b);
+    batch_reset_kernel<<<1, 64, 0, b->stream>>>(b->ticket, b->error, b->rec_cap > 0 ? b->rec_count : nullptr);
+    if (a.n_seg == 0) return ST_OK;
+    const size_t lds = lds_bytes(b->lib->precision, slots, b->hist_words);
+    const void *k = pick_kernel(b->lib->precision, slots);
+    void *args[] = {&a};
+    ST_HIP_TRY(hipLaunchKernel(k, dim3(grid), dim3(kBT), args, lds, b->stream));
+    return ST_OK;
+}
+
+int configure(st_batch *b, int slots, int bpc)
+{
+    const size_t lds = lds_bytes(b->lib->precision, slots, b->hist_words);
+    if (lds > 160 * 1024)
+        return set_error(ST_ERANGE, "batched kernel needs %zu B of LDS (limit 160 KiB): too many bins", lds);
+    const void *k = pick_kernel(b->lib->precision, slots);
+    ST_HIP_TRY(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    int per_cu = 0;
+    ST_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, kBT, lds));
+    if (per_cu < 1) return set_error(ST_EHIP, "batched kernel cannot be resident with %zu B LDS", lds);
+    b->slots = slots;
+    b->blocks_per_cu = std::max(1, std::min(bpc, per_cu));
+    b->grid = b->blocks_per_cu * device_cu_count();
+    b->lds = lds;
+    return ST_OK;
+}
+
+// Exact per-segment path for tracks the fused kernel does not take.
+int run_exact(st_batch *b)
+{
+    st_library *L = b->lib;
+    for (int32_t t : L->exact) {
+        const TrackDesc &td = L->tracks[t];
+        const int64_t n = td.n;
+        if (n > b->scratch.capacity) {
+            st_track_free(&b->scratch);
+            int rc = st_track_alloc(&b->scratch, n);
+            if (rc) return rc;
+        }
+        if (n > b->pairs.capacity) {
+            st_pairs_free(&b->pairs);
+            int rc = st_pairs_alloc(&b->pairs, n);
+            if (rc) return rc;
+        }
+        const size_t bytes = (size_t)n * sizeof(double);
+        ST_HIP_TRY(hipMemcpyAsync(b->scratch.x, L->x64 + td.offset, bytes, hipMemcpyDeviceToDevice, b->stream));
+        ST_HIP_TRY(hipMemcpyAsync(b->scratch.y, L->y64 + td.offset, bytes, hipMemcpyDeviceToDevice, b->stream));
+        ST_HIP_TRY(hipMemcpyAsync(b->scratch.z, L->z64 + td.offset, bytes, hipMemcpyDeviceToDevice, b->stream));
+        ST_HIP_TRY(hipMemcpyAsync(b->scratch.edep, L->e64 + td.offset, bytes, hipMemcpyDeviceToDevice, b->stream));
+        b->scratch.n = n;
+        st_pairs view = b->pairs;
+        view.capacity = std::max<int64_t>(n, 1);
+        st_vcs *m = b->methods[td.geom];
+        int rc = st_vcs_begin_track(m, &b->scratch, b->opts.seed, b->opts.track_index_base + (uint32_t)t,
+                                    b->opts.n_oversamples);
+        if (rc) return rc;
+        for (int os = 0; os < b->opts.n_oversamples; ++os) {
+            rc = st_vcs_process_oversample(m, &b->scratch, &view);
+            if (rc) return rc;
+            for (st_hist *h : b->hists) {
+                rc = hist_add_pairs(h, &view, b->stream);
+                if (rc) return rc;
+            }
+        }
+        m->in_track = false;
+    }
+    return ST_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int st_library_create(const double *x, const double *y, const double *z, const double *edep,
+                      const int64_t *offsets, int64_t n_tracks, const st_geometry *geoms, int32_t n_geoms,
+                      const int32_t *geom_of_track, int32_t precision, st_library **out)
+{
+    if (!out) return set_error(ST_EINVAL, "st_library_create: null output");
+    *out = nullptr;
+    if (n_tracks < 0 || !offsets || !geoms || n_geoms < 1 || n_geoms > kMaxGeoms)
+        return set_error(ST_EINVAL, "st_library_create: bad arguments (1 <= n_geoms <= %d)", kMaxGeoms);
+    if (precision != ST_F64 && precision != ST_F32)
+        return set_error(ST_EINVAL, "st_library_create: precision must be ST_F64 or ST_F32");
+    const int64_t n_points = offsets[n_tracks] - offsets[0];
+    if (n_points > 0 && (!x || !y || !z || !edep)) return set_error(ST_EINVAL, "st_library_create: null arrays");
+    for (int64_t t = 0; t < n_tracks; ++t) {
+        const int64_t n = offsets[t + 1] - offsets[t];
+        if (n < 0 || n > 0x7FFFFFFF) return set_error(ST_EINVAL, "st_library_create: bad offsets at track %lld",
+                                                      (long long)t);
+        if (geom_of_track && (geom_of_track[t] < 0 || geom_of_track[t] >= n_geoms))
+            return set_error(ST_EINVAL, "st_library_create: geometry index out of range at track %lld",
+                             (long long)t);
+    }
+    for (int i = 0; i < n_geoms; ++i)
+        if (!(geoms[i].diameter > 0) || geoms[i].n_linear < 1)
+            return set_error(ST_EINVAL, "st_library_create: invalid geometry %d", i);
+
+    st_library *L = new (std::nothrow) st_library;
+    if (!L) return set_error(ST_ENOMEM, "st_library_create: out of host memory");
+    L->n_tracks = n_tracks;
+    L->n_points = n_points;
+    L->precision = precision;
+    L->geoms.assign(geoms, geoms + n_geoms);
+    L->tracks.resize(n_tracks);
+
+    // block layout
+    int64_t nblocks = 0;
+    for (int64_t t = 0; t < n_tracks; ++t) {
+        TrackDesc &td = L->tracks[t];
+        td.offset = offsets[t] - offsets[0];
+        td.n = (int32_t)(offsets[t + 1] - offsets[t]);
+        td.nblocks = (td.n + kBlk - 1) / kBlk;
+        td.block0 = nblocks;
+        td.geom = geom_of_track ? geom_of_track[t] : 0;
+        td.flags = 0;
+        nblocks += td.nblocks;
+    }
+    L->n_blocks = nblocks;
+
+    std::vector<double> sx(n_points), sy(n_points), sz(n_points), se(n_points);
+    std::vector<double> zfirst(std::max<int64_t>(nblocks, 1));
+    std::vector<float> fx, fy, fz, fe;
+    std::vector<double> origin, eps;
+    if (precision == ST_F32) {
+        fx.resize(n_points);
+        fy.resize(n_points);
+        fz.resize(n_points);
+        fe.resize(n_points);
+        origin.resize(3 * std::max<int64_t>(nblocks, 1));
+        eps.resize(std::max<int64_t>(nblocks, 1));
+    }
+    const double *X = x, *Y = y, *Z = z, *E = edep;
+    const int64_t base = offsets[0];
+    parallel_for(n_tracks, [&](int64_t t) {
+        TrackDesc &td = L->tracks[t];
+        const int64_t o = td.offset, n = td.n;
+        std::vector<int64_t> idx(n);
+        std::iota(idx.begin(), idx.end(), 0);
+        const double *zt = Z + base + o;
+        auto zkey = [&](int64_t i) { return std::isnan(zt[i]) ? HUGE_VAL : zt[i]; };
+        std::stable_sort(idx.begin(), idx.end(), [&](int64_t a2, int64_t b2) { return zkey(a2) < zkey(b2); });
+        bool finite = true;
+        double xmin = HUGE_VAL, xmax = -HUGE_VAL, ymin = HUGE_VAL, ymax = -HUGE_VAL, esum = 0.0, emax = 0.0;
+        for (int64_t k = 0; k < n; ++k) {
+            const int64_t s = base + o + idx[k];
+            sx[o + k] = X[s];
+            sy[o + k] = Y[s];
+            sz[o + k] = Z[s];
+            se[o + k] = E[s];
+            finite = finite && std::isfinite(X[s]) && std::isfinite(Y[s]) && std::isfinite(Z[s]) &&
+                     std::isfinite(E[s]);
+            xmin = std::min(xmin, X[s]);
+            xmax = std::max(xmax, X[s]);
+            ymin = std::min(ymin, Y[s]);
+            ymax = std::max(ymax, Y[s]);
+            esum += std::fabs(E[s]);
+            emax = std::max(emax, std::fabs(E[s]));
+        }
+        td.xmin = xmin;
+        td.xmax = xmax;
+        td.ymin = ymin;
+        td.ymax = ymax;
+        td.zmax = n ? sz[o + n - 1] : 0.0;
+        for (int b = 0; b < td.nblocks; ++b) zfirst[td.block0 + b] = sz[o + (int64_t)b * kBlk];
+        const st_geometry &g = L->geoms[td.geom];
+        const double denom = (2. / 3.) * g.diameter;
+        // fixed-point headroom: sum of |y| over the track
+        const double ysum = esum / denom;
+        const double ylim = (precision == ST_F64) ? 2.7e11 : 1.0e9;
+        bool fast = finite && n <= kMaxFastPoints && ysum < ylim && emax / denom < ylim;
+        if (fast) {
+            const double w = 4.0 * g.diameter * (1.0 + 1e-12) + 1e-9;
+            const int64_t W = window_bound(sx.data() + o, sy.data() + o, sz.data() + o, n, g.diameter, w, 5);
+            fast = W <= kMaxWindow;
+        }
+        td.flags = fast ? 0u : kFlagExact;
+        if (precision == ST_F32) {
+            for (int b = 0; b < td.nblocks; ++b) {
+                const int64_t p0 = o + (int64_t)b * kBlk, p1 = std::min<int64_t>(p0 + kBlk, o + n);
+                double lo3[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL}, hi3[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+                const double *arr[3] = {sx.data(), sy.data(), sz.data()};
+                for (int64_t p = p0; p < p1; ++p)
+                    for (int a3 = 0; a3 < 3; ++a3) {
+                        lo3[a3] = std::min(lo3[a3], arr[a3][p]);
+                        hi3[a3] = std::max(hi3[a3], arr[a3][p]);
+                    }
+                double org[3];
+                for (int a3 = 0; a3 < 3; ++a3) {
+                    org[a3] = (std::isfinite(lo3[a3]) && std::isfinite(hi3[a3])) ? 0.5 * (lo3[a3] + hi3[a3]) : 0.0;
+                    origin[3 * (td.block0 + b) + a3] = org[a3];
+                }
+                double err = 0.0;
+                float *farr[3] = {fx.data(), fy.data(), fz.data()};
+                for (int64_t p = p0; p < p1; ++p) {
+                    for (int a3 = 0; a3 < 3; ++a3) {
+                        const double d = arr[a3][p] - org[a3];
+                        const float f = (float)d;
+                        farr[a3][p] = f;
+                        const double e1 = std::fabs((double)f - d) + std::fabs(d) * 2.3e-16;
+                        if (std::isfinite(e1)) err = std::max(err, e1);
+                        else err = HUGE_VAL;
+                    }
+                    fe[p] = (float)se[p];
+                }
+                eps[td.block0 + b] = err;
+            }
+        }
+    });
+
+    for (int64_t t = 0; t < n_tracks; ++t) {
+        if (L->tracks[t].flags & kFlagExact) L->exact.push_back((int32_t)t);
+        else L->fast_order.push_back((int32_t)t);
+    }
+    std::stable_sort(L->fast_order.begin(), L->fast_order.end(),
+                     [&](int32_t a2, int32_t b2) { return L->tracks[a2].n > L->tracks[b2].n; });
+
+    std::vector<GeomDev> gd(n_geoms);
+    for (int i = 0; i < n_geoms; ++i) {
+        const st_geometry &g = L->geoms[i];
+        gd[i].H = g.half_length;
+        gd[i].L = g.length;
+        gd[i].D = g.diameter;
+        gd[i].invD = 1.0 / g.diameter;
+        gd[i].r2 = g.radius * g.radius;
+        gd[i].gso = g.gso;
+        gd[i].denom = (2. / 3.) * g.diameter;
+        gd[i].denom_inv = 1.0 / gd[i].denom;
+        gd[i].N = (unsigned long long)g.n_linear;
+        gd[i].N2 = gd[i].N * gd[i].N;
+    }
+
+    int rc = ST_OK;
+    int64_t bytes = 0;
+    if (!rc) rc = dev_upload(&L->x64, sx.data(), n_points, &bytes);
+    if (!rc) rc = dev_upload(&L->y64, sy.data(), n_points, &bytes);
+    if (!rc) rc = dev_upload(&L->z64, sz.data(), n_points, &bytes);
+    if (!rc) rc = dev_upload(&L->e64, se.data(), n_points, &bytes);
+    if (!rc) rc = dev_upload(&L->zfirst, zfirst.data(), zfirst.size(), &bytes);
+    if (!rc && precision == ST_F32) {
+        rc = dev_upload(&L->dx32, fx.data(), n_points, &bytes);
+        if (!rc) rc = dev_upload(&L->dy32, fy.data(), n_points, &bytes);
+        if (!rc) rc = dev_upload(&L->dz32, fz.data(), n_points, &bytes);
+        if (!rc) rc = dev_upload(&L->e32, fe.data(), n_points, &bytes);
+        if (!rc) rc = dev_upload(&L->origin, origin.data(), origin.size(), &bytes);
+        if (!rc) rc = dev_upload(&L->eps, eps.data(), eps.size(), &bytes);
+    }
+    if (!rc) rc = dev_upload(&L->tracks_dev, L->tracks.data(), L->tracks.size(), &bytes);
+    if (!rc) rc = dev_upload(&L->geoms_dev, gd.data(), gd.size(), &bytes);
+    L->device_bytes = bytes;
+    if (rc) {
+        st_library_destroy(L);
+        return rc;
+    }
+    *out = L;
+    return ST_OK;
+}
+
+int st_library_info_get(const st_library *L, st_library_info *info)
+{
+    if (!L || !info) return set_error(ST_EINVAL, "st_library_info_get: null argument");
+    info->n_tracks = L->n_tracks;
+    info->n_points = L->n_points;
+    info->device_bytes = L->device_bytes;
+    info->fast_tracks = (int64_t)L->fast_order.size();
+    info->exact_tracks = (int64_t)L->exact.size();
+    info->precision = L->precision;
+    info->n_geometries = (int32_t)L->geoms.size();
+    return ST_OK;
+}
+
+void st_library_destroy(st_library *L)
+{
+    if (!L) return;
+    (void)hipDeviceSynchronize();
+    void *ptrs[] = {L->x64, L->y64, L->z64, L->e64, L->dx32, L->dy32, L->dz32, L->e32,
+                    L->zfirst, L->origin, L->eps, L->tracks_dev, L->geoms_dev};
+    for (void *p : ptrs) (void)hipFree(p);
+    delete L;
+}
+
+int st_batch_create(st_library *lib, const st_batch_opts *opts, st_hist *const *hists, int32_t n_hists,
+                    st_batch **out)
+{
+    if (!out) return set_error(ST_EINVAL, "st_batch_create: null output");
+    *out = nullptr;
+    if (!lib || !opts || (n_hists > 0 && !hists) || n_hists < 1 || n_hists > kMaxHists)
+        return set_error(ST_EINVAL, "st_batch_create: need a library and 1..%d histograms", kMaxHists);
+    if (opts->n_oversamples < 0) return set_error(ST_EINVAL, "st_batch_create: n_oversamples < 0");
+    for (int i = 0; i < n_hists; ++i)
+        if (!hists[i]) return set_error(ST_EINVAL, "st_batch_create: null histogram %d", i);
+    st_batch *b = new (std::nothrow) st_batch;
+    if (!b) return set_error(ST_ENOMEM, "st_batch_create: out of host memory");
+    b->lib = lib;
+    b->opts = *opts;
+    b->hists.assign(hists, hists + n_hists);
+    b->stream = default_stream();
+    for (st_hist *h : b->hists) b->hist_words += h->nbins;
+    int rc = ST_OK;
+    auto fail = [&](int code) {
+        st_batch_destroy(b);
+        return code;
+    };
+    hipError_t e = hipMalloc((void **)&b->ticket, sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc((void **)&b->error, sizeof(int));
+    if (e == hipSuccess) e = hipMalloc((void **)&b->rec_count, sizeof(unsigned long long));
+    if (e == hipSuccess && !lib->fast_order.empty())
+        e = hipMalloc((void **)&b->order_dev, lib->fast_order.size() * sizeof(int32_t));
+    if (e == hipSuccess && !lib->fast_order.empty())
+        e = hipMemcpy(b->order_dev, lib->fast_order.data(), lib->fast_order.size() * sizeof(int32_t),
+                      hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(b->rec_count, 0, sizeof(unsigned long long));
+    if (e != hipSuccess) return fail(set_error(ST_EHIP, "st_batch_create: %s", hipGetErrorString(e)));
+    for (const st_geometry &g : lib->geoms) {
+        st_vcs *m = nullptr;
+        rc = st_vcs_create(&g, opts->shift_xy, opts->shift_z, 0, 0, &m);
+        if (rc) return fail(rc);
+        st_vcs_set_stream(m, b->stream);
+        b->methods.push_back(m);
+    }
+    int slots = opts->table_slots ? opts->table_slots : 2048;
+    if (slots != 1024 && slots != 2048 && slots != 4096)
+        return fail(set_error(ST_EINVAL, "st_batch_create: table_slots must be 1024, 2048 or 4096"));
+    int bpc = opts->blocks_per_cu ? opts->blocks_per_cu : 4;
+    rc = configure(b, slots, bpc);
+    if (rc) return fail(rc);
+    if (opts->autotune && !lib->fast_order.empty() && opts->n_oversamples > 0) {
+        // time each candidate on one full pass; keep the fastest
+        const int cand_slots[] = {1024, 2048, 4096};
+        const int cand_bpc[] = {2, 3, 4, 6, 8};
+        hipEvent_t e0, e1;
+        ST_HIP_TRY(hipEventCreate(&e0));
+        ST_HIP_TRY(hipEventCreate(&e1));
+        float best = 1e30f;
+        int best_s = slots, best_b = b->blocks_per_cu;
+        std::vector<std::vector<uint64_t>> saved;
+        for (st_hist *h : b->hists) {
+            saved.emplace_back(h->nbins);
+            ST_HIP_TRY(hipMemcpy(saved.back().data(), h->bins_dev, h->nbins * sizeof(uint64_t),
+                                 hipMemcpyDeviceToHost));
+        }
+        for (int s : cand_slots) {
+            for (int bp : cand_bpc) {
+                if (configure(b, s, bp) != ST_OK) continue;
+                if (b->blocks_per_cu != bp) continue;
+                ST_HIP_TRY(hipEventRecord(e0, b->stream));
+                rc = launch_fast(b, b->slots, b->grid);
+                if (rc) return fail(rc);
+                ST_HIP_TRY(hipEventRecord(e1, b->stream));
+                ST_HIP_TRY(hipEventSynchronize(e1));
+                float ms = 0;
+                ST_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+                if (ms < best) {
+                    best = ms;
+                    best_s = s;
+                    best_b = bp;
+                }
+            }
+        }
+        for (size_t i = 0; i < b->hists.size(); ++i)
+            ST_HIP_TRY(hipMemcpy(b->hists[i]->bins_dev, saved[i].data(), saved[i].size() * sizeof(uint64_t),
+                                 hipMemcpyHostToDevice));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        rc = configure(b, best_s, best_b);
+        if (rc) return fail(rc);
+    }
+    *out = b;
+    return ST_OK;
+}
+
+int st_batch_set_stream(st_batch *b, void *hip_stream)
+{
+    if (!b) return set_error(ST_EINVAL, "st_batch_set_stream: null batch");
+    b->stream = hip_stream ? (hipStream_t)hip_stream : default_stream();
+    for (st_vcs *m : b->methods) st_vcs_set_stream(m, b->stream);
+    if (b->exec) {
+        (void)hipGraphExecDestroy(b->exec);
+        b->exec = nullptr;
+    }
+    if (b->graph) {
+        (void)hipGraphDestroy(b->graph);
+        b->graph = nullptr;
+    }
+    b->graph_ready = false;
+    return ST_OK;
+}
+
+int st_batch_run(st_batch *b)
+{
+    if (!b) return set_error(ST_EINVAL, "st_batch_run: null batch");
+    if (b->opts.n_oversamples == 0) return ST_OK;
+    if (b->opts.use_graph && b->rec_cap == 0) {
+        if (!b->graph_ready) {
+            ST_HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            int rc = launch_fast(b, b->slots, b->grid);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(b->stream, &g);
+            if (rc) return rc;
+            if (e != hipSuccess) return set_error(ST_EHIP, "stream capture failed: %s", hipGetErrorString(e));
+            b->graph = g;
+            ST_HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+            b->graph_ready = true;
+        }
+        ST_HIP_TRY(hipGraphLaunch(b->exec, b->stream));
+    } else {
+        int rc = launch_fast(b, b->slots, b->grid);
+        if (rc) return rc;
+    }
+    return run_exact(b);
+}
+
+int st_batch_sync(st_batch *b)
+{
+    if (!b) return set_error(ST_EINVAL, "st_batch_sync: null batch");
+    ST_HIP_TRY(hipStreamSynchronize(b->stream));
+    int err = 0;
+    ST_HIP_TRY(hipMemcpy(&err, b->error, sizeof err, hipMemcpyDeviceToHost));
+    if (err) return set_error(ST_ESTATE, "batched kernel reported a full sphere table (code %d)", err);
+    return ST_OK;
+}
+
+int st_batch_params(const st_batch *b, int32_t *p)
+{
+    if (!b || !p) return set_error(ST_EINVAL, "st_batch_params: null argument");
+    p[0] = b->threads;
+    p[1] = b->slots;
+    p[2] = b->blocks_per_cu;
+    return ST_OK;
+}
+
+int st_batch_record(st_batch *b, int64_t capacity)
+{
+    if (!b || capacity < 0) return set_error(ST_EINVAL, "st_batch_record: bad arguments");
+    (void)hipFree(b->rec_seg);
+    (void)hipFree(b->rec_key);
+    (void)hipFree(b->rec_sum);
+    b->rec_seg = nullptr;
+    b->rec_key = nullptr;
+    b->rec_sum = nullptr;
+    b->rec_cap = 0;
+    if (capacity == 0) return ST_OK;
+    ST_HIP_TRY(hipMalloc((void **)&b->rec_seg, capacity * sizeof(int64_t)));
+    ST_HIP_TRY(hipMalloc((void **)&b->rec_key, capacity * sizeof(unsigned long long)));
+    ST_HIP_TRY(hipMalloc((void **)&b->rec_sum, capacity * sizeof(double)));
+    b->rec_cap = capacity;
+    return ST_OK;
+}
+
+int st_batch_record_fetch(st_batch *b, int64_t *segments, uint64_t *keys, double *sums, int64_t *count)
+{
+    if (!b || !count) return set_error(ST_EINVAL, "st_batch_record_fetch: null argument");
+    ST_HIP_TRY(hipStreamSynchronize(b->stream));
+    unsigned long long c = 0;
+    ST_HIP_TRY(hipMemcpy(&c, b->rec_count, sizeof c, hipMemcpyDeviceToHost));
+    *count = (int64_t)c;
+    if ((int64_t)c > b->rec_cap) return set_error(ST_ERANGE, "st_batch_record_fetch: %llu records, capacity %lld",
+                                                 c, (long long)b->rec_cap);
+    if (c && segments) ST_HIP_TRY(hipMemcpy(segments, b->rec_seg, c * sizeof(int64_t), hipMemcpyDeviceToHost));
+    if (c && keys) ST_HIP_TRY(hipMemcpy(keys, b->rec_key, c * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    if (c && sums) ST_HIP_TRY(hipMemcpy(sums, b->rec_sum, c * sizeof(double), hipMemcpyDeviceToHost));
+    return ST_OK;
+}
+
+void st_batch_destroy(st_batch *b)
+{
+    if (!b) return;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    for (st_vcs *m : b->methods) st_vcs_destroy(m);
+    st_track_free(&b->scratch);
+    st_pairs_free(&b->pairs);
+    void *ptrs[] = {b->order_dev, b->ticket, b->error, b->rec_count, b->rec_seg, b->rec_key, b->rec_sum};
+    for (void *p : ptrs) (void)hipFree(p);
+    delete b;
+}
+
+}  // extern "C"

@@ -1,0 +1,1056 @@
+// plain.hip -- the per-track drop-in path on gfx950: Track / VolumeEdepPair
+// buffers, VoxelConstrainedSphereMethod (shift table, z offset, one fused
+// shift -> box -> sphere -> key kernel per oversample) and Histogram
+// (stable radix sort -> left-to-right reduce-by-key -> bin).
+//
+// Results are bit-identical to the FP64 restatement of the reference kernels
+// (VoxelConstrainedSphereMethod.cu:156-324, Histogram.cu:147-171) with
+// per-sphere sums taken in ascending point index: compaction preserves point
+// order and the sort is stable.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+
+#include "st_common.hpp"
+#include "st_internal.hpp"
+
+namespace st {
+namespace {
+
+// ---------------------------------------------------------------------------
+// exact nearest lattice index
+// ---------------------------------------------------------------------------
+// The reference takes llrint(a / D) with a correctly rounded division.  a * invD
+// differs from fl(a / D) by at most 3.0001 * 2^-53 * |a / D|, so the two round
+// to the same integer unless q lies within that distance of a half-integer;
+// only then is the division performed.  Either way the result equals
+// llrint(fl(a / D)) exactly.
+__device__ __forceinline__ double nearest_index(double a, double D, double invD)
+{
+    const double q = a * invD;
+    double k = rint(q);
+    const double f = fabs(q - k);   // exact: q and k are within a factor of 2 or k == 0
+    if (f > 0.5 - (fabs(q) * 1e-15 + 1e-300)) k = rint(a / D);
+    return k;
+}
+
+// ---------------------------------------------------------------------------
+// per-track kernels
+// ---------------------------------------------------------------------------
+// ShiftTrackZOrigin (VoxelConstrainedSphereMethod.cu:138-154).
+__global__ void zshift_kernel(double *__restrict__ z, int64_t n, double half_length)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        z[i] = z[i] + half_length;
+}
+
+// e / ((2/3) D) (VoxelConstrainedSphereMethod.cu:302,322) depends only on the
+// point, so it is computed once per track instead of once per oversample.
+__global__ void lineal_kernel(const double *__restrict__ e, double *__restrict__ lin, int64_t n,
+                              double denominator)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        lin[i] = e[i] / denominator;
+}
+
+// GenerateRandomXYShift (VoxelConstrainedSphereMethod.cu:61-83) with Philox.
+__global__ void shift_table_kernel(double *__restrict__ shifts, int n_os, uint64_t seed,
+                                   uint32_t track_index, int shift_xy, int shift_z, double length,
+                                   double half_length)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_os) return;
+    const Shift s = shift_for(seed, track_index, (uint32_t)k, shift_xy, shift_z, length, half_length);
+    shifts[3 * k + 0] = s.x;
+    shifts[3 * k + 1] = s.y;
+    shifts[3 * k + 2] = s.z;
+}
+
+// ---------------------------------------------------------------------------
+// fused scoring kernel with order-preserving decoupled look-back compaction
+// ---------------------------------------------------------------------------
+constexpr int kScoreBlock = 256;
+constexpr int kScoreRows = 4;
+constexpr int kScoreTile = kScoreBlock * kScoreRows;
+constexpr int kWaves = kScoreBlock / 64;
+
+struct ScoreArgs {
+    const double *x, *y, *z, *lin;
+    int64_t n;
+    double H, D, invD, gso, r2;
+    uint64_t N;
+    const double *shifts;
+    int os;
+    uint64_t *out_key;
+    double *out_y;
+    int32_t *out_count;
+    unsigned long long *status;
+    unsigned long long *ticket;
+    unsigned long long ticket_base;
+    uint32_t epoch;
+    int num_tiles;
+};
+
+// status word: [63:62] flag (1 aggregate, 2 inclusive prefix), [61:32] epoch,
+// [31:0] count.  Epochs make stale words from earlier calls unreadable, so the
+// array is never cleared.
+__device__ __forceinline__ unsigned long long pack_status(unsigned long long flag, uint32_t epoch,
+                                                          uint32_t v)
+{
+    return (flag << 62) | ((unsigned long long)(epoch & 0x3FFFFFFFu) << 32) | v;
+}
+
+__global__ __launch_bounds__(kScoreBlock) void score_kernel(ScoreArgs a)
+{
+    __shared__ int s_tile;
+    __shared__ uint32_t s_counts[kScoreRows][kWaves];
+    __shared__ uint32_t s_excl;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    if (tid == 0)
+        s_tile = (int)(atomicAdd(a.ticket, 1ull) - a.ticket_base);
+    __syncthreads();
+    const int tile = s_tile;
+    const int64_t base = (int64_t)tile * kScoreTile;
+    const double sx = a.shifts[3 * a.os + 0];
+    const double sy = a.shifts[3 * a.os + 1];
+    const double sz = a.shifts[3 * a.os + 2];
+
+    bool flag[kScoreRows];
+    uint64_t key[kScoreRows];
+    double yv[kScoreRows];
+#pragma unroll
+    for (int r = 0; r < kScoreRows; ++r) {
+        flag[r] = false;
+        key[r] = 0;
+        yv[r] = 0.0;
+        const int64_t i = base + (int64_t)r * kScoreBlock + tid;
+        if (i < a.n) {
+            // FilterInScoringBox (:183-198)
+            const double xs = a.x[i] + sx;
+            const double ys = a.y[i] + sy;
+            const double zs = a.z[i] + sz;
+            if (fabs(xs) < a.H && fabs(ys) < a.H && fabs(zs) < a.H) {
+                // FilterTrackInSphere (:249-271)
+                const double ax = xs - a.gso;
+                const double ay = ys - a.gso;
+                const double az = zs - a.gso;
+                const double kx = nearest_index(ax, a.D, a.invD);
+                const double ky = nearest_index(ay, a.D, a.invD);
+                const double kz = nearest_index(az, a.D, a.invD);
+                const double dx = (kx * a.D) - ax;
+                const double dy = (ky * a.D) - ay;
+                const double dz = (kz * a.D) - az;
+                const double dist = ((dx * dx) + (dy * dy)) + (dz * dz);
+                if (dist <= a.r2) {
+                    // ScoreTrackInSphere (:313-322)
+                    const uint64_t ix = (uint64_t)(int64_t)kx;
+                    const uint64_t iy = (uint64_t)(int64_t)ky;
+                    const uint64_t iz = (uint64_t)(int64_t)kz;
+                    key[r] = ix + iy * a.N + iz * a.N * a.N;
+                    yv[r] = a.lin[i];
+                    flag[r] = true;
+                }
+            }
+        }
+    }
+
+    // ranks inside the tile (ascending point index = row-major order)
+    uint32_t rank[kScoreRows];
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
+    for (int r = 0; r < kScoreRows; ++r) {
+        const uint64_t m = __ballot(flag[r]);
+        rank[r] = (uint32_t)__popcll(m & lt_mask);
+        if (lane == 0) s_counts[r][wave] = (uint32_t)__popcll(m);
+    }
+    __syncthreads();
+    uint32_t tile_total = 0;
+    uint32_t off[kScoreRows];
+#pragma unroll
+    for (int r = 0; r < kScoreRows; ++r) {
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) {
+            if (w == wave) off[r] = tile_total;
+            tile_total += s_counts[r][w];
+        }
+    }
+
+    if (tid == 0) {
+        uint32_t excl = 0;
+        if (tile == 0) {
+            __hip_atomic_store(&a.status[0], pack_status(2, a.epoch, tile_total), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            __hip_atomic_store(&a.status[tile], pack_status(1, a.epoch, tile_total),
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            int j = tile - 1;
+            const uint32_t ep = a.epoch & 0x3FFFFFFFu;
+            while (true) {
+                const unsigned long long s =
+                    __hip_atomic_load(&a.status[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const uint32_t sep = (uint32_t)(s >> 32) & 0x3FFFFFFFu;
+                const unsigned long long fl = s >> 62;
+                if (sep != ep || fl == 0) {
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                excl += (uint32_t)s;
+                if (fl == 2) break;
+                --j;
+            }
+            __hip_atomic_store(&a.status[tile], pack_status(2, a.epoch, excl + tile_total),
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        s_excl = excl;
+        if (tile == a.num_tiles - 1) *a.out_count = (int32_t)(excl + tile_total);
+    }
+    __syncthreads();
+    const uint32_t excl = s_excl;
+#pragma unroll
+    for (int r = 0; r < kScoreRows; ++r) {
+        if (flag[r]) {
+            const uint32_t pos = excl + off[r] + rank[r];
+            a.out_key[pos] = key[r];
+            a.out_y[pos] = yv[r];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Histogram: stable LSD radix sort -> left-to-right reduce-by-key -> bin
+// ---------------------------------------------------------------------------
+// Small inputs (m <= kLdsCap pairs) are sorted by one 1024-thread workgroup in
+// LDS with 4-bit digits; larger inputs by all workgroups of a cooperative
+// launch with 8-bit digits and grid barriers between phases.  Both sort only
+// as many bits as the largest key present has.
+constexpr int kSortThreads = 1024;
+constexpr int kSortWaves = kSortThreads / 64;
+constexpr int kLdsCap = 6144;
+constexpr int kDigitBits = 4;
+constexpr int kDigits = 1 << kDigitBits;
+constexpr size_t kLdsBytes = (size_t)kLdsCap * (2 * sizeof(uint64_t) + 2 * sizeof(uint16_t)) +
+                             (size_t)kDigits * kSortThreads * sizeof(uint16_t);
+constexpr int kGBits = 8;
+constexpr int kGDigits = 1 << kGBits;
+
+struct HistArgs {
+    const uint64_t *keys;
+    const double *y;
+    const int32_t *count;
+    const double *edges;
+    BinGuess guess;
+    unsigned long long *bins;
+};
+
+// Exclusive scan of one value per thread over a 1024-thread workgroup.
+__device__ __forceinline__ uint32_t block_exclusive_scan_1024(uint32_t v, uint32_t *s_wave)
+{
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    uint32_t incl = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t t = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += t;
+    }
+    if (lane == 63) s_wave[wave] = incl;
+    __syncthreads();
+    if (wave == 0) {
+        const uint32_t w = (lane < kSortWaves) ? s_wave[lane] : 0u;
+        uint32_t wi = w;
+#pragma unroll
+        for (int d = 1; d < kSortWaves; d <<= 1) {
+            const uint32_t t = __shfl_up(wi, d, 64);
+            if (lane >= d) wi += t;
+        }
+        if (lane < kSortWaves) s_wave[lane] = wi - w;
+    }
+    __syncthreads();
+    const uint32_t r = s_wave[wave] + incl - v;
+    __syncthreads();
+    return r;
+}
+
+__device__ __forceinline__ void bin_sum(const HistArgs &a, double s)
+{
+    const int b = bin_of(a.edges, a.guess.nbins, s, bin_guess(a.guess, s));
+    if (b >= 0) atomicAdd(&a.bins[b], 1ull);
+}
+
+// One workgroup: sort m <= kLdsCap pairs in LDS, reduce, bin.
+__device__ void lds_sort_reduce_bin(const HistArgs &a, int m, unsigned char *smem)
+{
+    uint64_t *k0 = reinterpret_cast<uint64_t *>(smem);
+    uint64_t *k1 = k0 + kLdsCap;
+    uint16_t *v0 = reinterpret_cast<uint16_t *>(k1 + kLdsCap);
+    uint16_t *v1 = v0 + kLdsCap;
+    uint16_t *cnt = v1 + kLdsCap;   // [kDigits][kSortThreads]
+    __shared__ uint32_t s_wave[kSortWaves];
+    __shared__ unsigned long long s_max;
+    const int tid = threadIdx.x;
+    if (tid == 0) s_max = 0;
+    __syncthreads();
+    unsigned long long my_max = 0;
+    for (int i = tid; i < m; i += kSortThreads) {
+        const uint64_t kk = a.keys[i];
+        k0[i] = kk;
+        v0[i] = (uint16_t)i;
+        my_max = kk > my_max ? kk : my_max;
+    }
+    atomicMax(&s_max, my_max);
+    __syncthreads();
+    const unsigned long long kmax = s_max;
+    const int bits = kmax ? 64 - __clzll((long long)kmax) : 0;
+    const int passes = (bits + kDigitBits - 1) / kDigitBits;
+    const int ipt = (m + kSortThreads - 1) / kSortThreads;
+    const int my_lo = min(tid * ipt, m);
+    const int my_hi = min(my_lo + ipt, m);
+    uint64_t *src_k = k0, *dst_k = k1;
+    uint16_t *src_v = v0, *dst_v = v1;
+    for (int p = 0; p < passes; ++p) {
+        const int shift = p * kDigitBits;
+        uint32_t c[kDigits];
+#pragma unroll
+        for (int d = 0; d < kDigits; ++d) c[d] = 0;
+        for (int i = my_lo; i < my_hi; ++i) {
+            const int d = (int)((src_k[i] >> shift) & (kDigits - 1));
+#pragma unroll
+            for (int dd = 0; dd < kDigits; ++dd) c[dd] += (d == dd);
+        }
+#pragma unroll
+        for (int d = 0; d < kDigits; ++d) cnt[d * kSortThreads + tid] = (uint16_t)c[d];
+        __syncthreads();
+        // exclusive scan of the digit-major table; thread t scans entries
+        // [t * kDigits, (t + 1) * kDigits)
+        uint32_t run = 0;
+        uint32_t local[kDigits];
+#pragma unroll
+        for (int j = 0; j < kDigits; ++j) {
+            local[j] = run;
+            run += cnt[tid * kDigits + j];
+        }
+        const uint32_t base = block_exclusive_scan_1024(run, s_wave);
+#pragma unroll
+        for (int j = 0; j < kDigits; ++j) cnt[tid * kDigits + j] = (uint16_t)(base + local[j]);
+        __syncthreads();
+        for (int i = my_lo; i < my_hi; ++i) {
+            const uint64_t kk = src_k[i];
+            const int d = (int)((kk >> shift) & (kDigits - 1));
+            const uint32_t pos = cnt[d * kSortThreads + tid];
+            cnt[d * kSortThreads + tid] = (uint16_t)(pos + 1);
+            dst_k[pos] = kk;
+            dst_v[pos] = src_v[i];
+        }
+        __syncthreads();
+        uint64_t *tk = src_k; src_k = dst_k; dst_k = tk;
+        uint16_t *tv = src_v; src_v = dst_v; dst_v = tv;
+    }
+    // reduce-by-key left to right (Histogram.cu:153), bin (:158)
+    for (int i = tid; i < m; i += kSortThreads) {
+        const uint64_t kk = src_k[i];
+        if (i > 0 && src_k[i - 1] == kk) continue;
+        double s = a.y[src_v[i]];
+        for (int j = i + 1; j < m && src_k[j] == kk; ++j) s = s + a.y[src_v[j]];
+        bin_sum(a, s);
+    }
+}
+
+__global__ __launch_bounds__(kSortThreads) void hist_lds_kernel(HistArgs a)
+{
+    extern __shared__ unsigned char smem[];
+    const int m = *a.count;
+    if (m <= 0 || m > kLdsCap) return;
+    lds_sort_reduce_bin(a, m, smem);
+}
+
+// Grid barrier for a cooperative launch (all workgroups co-resident).
+// bar[0] counts arrivals, bar[1] is the generation.
+__device__ __forceinline__ void grid_barrier(unsigned int *bar, unsigned int nblocks)
+{
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned int gen = __hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence();
+        const unsigned int arrived =
+            __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (arrived == nblocks - 1) {
+            __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&bar[1], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            while (__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen)
+                __builtin_amdgcn_s_sleep(2);
+        }
+        __threadfence();
+    }
+    __syncthreads();
+}
+
+struct CoopArgs {
+    HistArgs h;
+    uint64_t *kbuf[2];
+    uint32_t *vbuf[2];
+    uint32_t *block_counts;          // [kGDigits][gridDim.x]
+    unsigned long long *kmax;        // zero between launches
+    unsigned int *barrier;           // [2]
+};
+
+// All workgroups: m > kLdsCap pairs.  Workgroup b owns positions
+// [b * chunk, (b + 1) * chunk) in every pass; rows of kSortThreads positions
+// are ranked stably with wave ballots (8-bit multi-split).
+__global__ __launch_bounds__(kSortThreads) void hist_coop_kernel(CoopArgs a)
+{
+    extern __shared__ unsigned char smem[];
+    const int m = *a.h.count;
+    if (m <= kLdsCap) {
+        if (m > 0 && blockIdx.x == 0) lds_sort_reduce_bin(a.h, m, smem);
+        return;   // uniform over the grid: no barrier is reached
+    }
+    const unsigned int G = gridDim.x;
+    const int b = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    uint32_t *s_hist = reinterpret_cast<uint32_t *>(smem);             // [kGDigits]
+    uint32_t *s_run = s_hist + kGDigits;                                 // [kGDigits]
+    uint16_t *s_wcnt = reinterpret_cast<uint16_t *>(s_run + kGDigits);  // [kSortWaves][kGDigits]
+    uint32_t *s_wave = reinterpret_cast<uint32_t *>(s_wcnt + kSortWaves * kGDigits);
+
+    int64_t chunk = ((int64_t)m + G - 1) / G;
+    chunk = (chunk + kSortThreads - 1) / kSortThreads * kSortThreads;
+    const int64_t lo = min((int64_t)m, (int64_t)b * chunk);
+    const int64_t hi = min((int64_t)m, lo + chunk);
+
+    // largest key -> number of 8-bit passes
+    {
+        unsigned long long mx = 0;
+        for (int64_t i = lo + tid; i < hi; i += kSortThreads) {
+            const uint64_t kk = a.h.keys[i];
+            mx = kk > mx ? kk : mx;
+        }
+        for (int d = 32; d > 0; d >>= 1) {
+            const unsigned long long o = __shfl_xor(mx, d, 64);
+            mx = o > mx ? o : mx;
+        }
+        if (lane == 0 && mx) atomicMax(a.kmax, mx);
+    }
+    grid_barrier(a.barrier, G);
+    const unsigned long long kmax = __hip_atomic_load(a.kmax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int bits = kmax ? 64 - __clzll((long long)kmax) : 0;
+    const int passes = (bits + kGBits - 1) / kGBits;
+
+    for (int i = tid; i < kSortWaves * kGDigits; i += kSortThreads) s_wcnt[i] = 0;
+    const uint64_t *src_k = nullptr;
+    const uint32_t *src_v = nullptr;
+    for (int p = 0; p < passes; ++p) {
+        const int shift = p * kGBits;
+        uint64_t *dst_k = a.kbuf[p & 1];
+        uint32_t *dst_v = a.vbuf[p & 1];
+        // phase A: digit counts of this workgroup's chunk
+        for (int d = tid; d < kGDigits; d += kSortThreads) s_hist[d] = 0;
+        __syncthreads();
+        for (int64_t i = lo + tid; i < hi; i += kSortThreads) {
+            const uint64_t kk = src_k ? src_k[i] : a.h.keys[i];
+            atomicAdd(&s_hist[(kk >> shift) & (kGDigits - 1)], 1u);
+        }
+        __syncthreads();
+        for (int d = tid; d < kGDigits; d += kSortThreads) a.block_counts[(size_t)d * G + b] = s_hist[d];
+        grid_barrier(a.barrier, G);
+        // phase B: workgroup 0 turns the digit-major table into exclusive offsets
+        if (b == 0) {
+            const int total = kGDigits * (int)G;
+            const int per = (total + kSortThreads - 1) / kSortThreads;
+            const int e0 = min(tid * per, total);
+            const int e1 = min(e0 + per, total);
+            uint32_t run = 0;
+            for (int e = e0; e < e1; ++e) run += a.block_counts[e];
+            uint32_t base = block_exclusive_scan_1024(run, s_wave);
+            for (int e = e0; e < e1; ++e) {
+                const uint32_t c = a.block_counts[e];
+                a.block_counts[e] = base;
+                base += c;
+            }
+        }
+        grid_barrier(a.barrier, G);
+        // phase C: stable scatter, one row of kSortThreads positions at a time
+        for (int d = tid; d < kGDigits; d += kSortThreads) s_run[d] = a.block_counts[(size_t)d * G + b];
+        __syncthreads();
+        const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+        for (int64_t r0 = lo; r0 < hi; r0 += kSortThreads) {
+            const int64_t i = r0 + tid;
+            const bool valid = i < hi;
+            uint64_t kk = 0;
+            uint32_t vv = 0;
+            if (valid) {
+                if (src_k) {
+                    kk = src_k[i];
+                    vv = src_v[i];
+                } else {
+                    kk = a.h.keys[i];
+                    vv = (uint32_t)i;
+                }
+            }
+            const uint32_t d = (uint32_t)((kk >> shift) & (kGDigits - 1));
+            uint64_t same = __ballot(valid);
+#pragma unroll
+            for (int bit = 0; bit < kGBits; ++bit) {
+                const uint64_t bb = __ballot((d >> bit) & 1u);
+                same &= ((d >> bit) & 1u) ? bb : ~bb;
+            }
+            const uint32_t rank = (uint32_t)__popcll(same & lt_mask);
+            if (valid && rank == 0) s_wcnt[wave * kGDigits + d] = (uint16_t)__popcll(same);
+            __syncthreads();
+            if (valid) {
+                uint32_t before = 0;
+                for (int w = 0; w < wave; ++w) before += s_wcnt[w * kGDigits + d];
+                const uint32_t pos = s_run[d] + before + rank;
+                dst_k[pos] = kk;
+                dst_v[pos] = vv;
+            }
+            __syncthreads();
+            for (int dd = tid; dd < kGDigits; dd += kSortThreads) {
+                uint32_t add = 0;
+                for (int w = 0; w < kSortWaves; ++w) {
+                    add += s_wcnt[w * kGDigits + dd];
+                    s_wcnt[w * kGDigits + dd] = 0;
+                }
+                s_run[dd] += add;
+            }
+            __syncthreads();
+        }
+        grid_barrier(a.barrier, G);
+        src_k = dst_k;
+        src_v = dst_v;
+    }
+    // reduce-by-key left to right and bin; runs start inside this chunk
+    for (int64_t i = lo + tid; i < hi; i += kSortThreads) {
+        const uint64_t kk = src_k ? src_k[i] : a.h.keys[i];
+        if (i > 0 && (src_k ? src_k[i - 1] : a.h.keys[i - 1]) == kk) continue;
+        double s = a.h.y[src_v ? src_v[i] : (uint32_t)i];
+        for (int64_t j = i + 1; j < m; ++j) {
+            const uint64_t kj = src_k ? src_k[j] : a.h.keys[j];
+            if (kj != kk) break;
+            s = s + a.h.y[src_v ? src_v[j] : (uint32_t)j];
+        }
+        bin_sum(a.h, s);
+    }
+    if (b == 0 && tid == 0) *a.kmax = 0;   // read by every workgroup before the first phase-A barrier
+}
+
+constexpr size_t kCoopScratchBytes = (size_t)kGDigits * 2 * sizeof(uint32_t) +
+                                     (size_t)kSortWaves * kGDigits * sizeof(uint16_t) +
+                                     (size_t)kSortWaves * sizeof(uint32_t);
+static_assert(kCoopScratchBytes <= kLdsBytes, "global-phase LDS must fit the LDS-path allocation");
+
+__global__ void zero_u64_kernel(unsigned long long *p, int64_t n)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0;
+}
+
+inline int grid_for(int64_t n, int threads, int cap = 65535)
+{
+    int64_t g = (n + threads - 1) / threads;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (int)g;
+}
+
+template <typename T>
+int grow(T **p, int64_t *cap, int64_t need)
+{
+    if (need <= *cap && *p) return ST_OK;
+    if (*p) {
+        (void)hipFree(*p);
+        *p = nullptr;
+        *cap = 0;
+    }
+    if (need <= 0) return ST_OK;
+    hipError_t e = hipMalloc((void **)p, (size_t)need * sizeof(T));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return set_error(ST_ENOMEM, "hipMalloc(%lld x %zu B) failed: %s", (long long)need, sizeof(T),
+                         hipGetErrorString(e));
+    }
+    *cap = need;
+    return ST_OK;
+}
+
+
+int ensure_sort_attributes()
+{
+    static bool done = false;
+    if (done) return ST_OK;
+    ST_HIP_TRY(hipFuncSetAttribute((const void *)hist_lds_kernel,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBytes));
+    ST_HIP_TRY(hipFuncSetAttribute((const void *)hist_coop_kernel,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBytes));
+    done = true;
+    return ST_OK;
+}
+
+// Buffers of the cooperative sort: ping-pong keys/payloads sized for `cap`
+// pairs, the digit table of the grid, the max-key cell and the barrier.
+int ensure_sort_buffers(st_hist *h, int64_t cap)
+{
+    int rc = ensure_sort_attributes();
+    if (rc) return rc;
+    if (!h->coop_grid) {
+        int per_cu = 0;
+        ST_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)hist_coop_kernel,
+                                                                kSortThreads, kLdsBytes));
+        if (per_cu < 1) return set_error(ST_EHIP, "hist_coop_kernel cannot be resident");
+        h->coop_grid = per_cu * device_cu_count();
+        ST_HIP_TRY(hipMalloc((void **)&h->block_counts, (size_t)kGDigits * h->coop_grid * sizeof(uint32_t)));
+        ST_HIP_TRY(hipMalloc((void **)&h->kmax, sizeof(unsigned long long)));
+        ST_HIP_TRY(hipMalloc((void **)&h->barrier, 2 * sizeof(unsigned int)));
+        ST_HIP_TRY(hipMemset(h->kmax, 0, sizeof(unsigned long long)));
+        ST_HIP_TRY(hipMemset(h->barrier, 0, 2 * sizeof(unsigned int)));
+    }
+    if (cap > h->sort_capacity) {
+        (void)hipFree(h->keys_a); (void)hipFree(h->keys_b); (void)hipFree(h->vals_a); (void)hipFree(h->vals_b);
+        h->keys_a = h->keys_b = nullptr;
+        h->vals_a = h->vals_b = nullptr;
+        h->sort_capacity = 0;
+        ST_HIP_TRY(hipMalloc((void **)&h->keys_a, (size_t)cap * sizeof(uint64_t)));
+        ST_HIP_TRY(hipMalloc((void **)&h->keys_b, (size_t)cap * sizeof(uint64_t)));
+        ST_HIP_TRY(hipMalloc((void **)&h->vals_a, (size_t)cap * sizeof(uint32_t)));
+        ST_HIP_TRY(hipMalloc((void **)&h->vals_b, (size_t)cap * sizeof(uint32_t)));
+        h->sort_capacity = cap;
+    }
+    return ST_OK;
+}
+
+}  // namespace
+
+// Sort -> reduce -> bin the pairs into h->bins_dev on `stream` (no zeroing).
+int hist_add_pairs(st_hist *h, const st_pairs *pairs, hipStream_t stream)
+{
+    int rc = ensure_sort_attributes();
+    if (rc) return rc;
+    HistArgs ha;
+    ha.keys = pairs->volume;
+    ha.y = pairs->edep;
+    ha.count = pairs->num_elements;
+    ha.edges = h->edges_dev;
+    ha.guess = h->guess;
+    ha.bins = (unsigned long long *)h->bins_dev;
+    if (pairs->capacity <= kLdsCap) {
+        hist_lds_kernel<<<1, kSortThreads, kLdsBytes, stream>>>(ha);
+        ST_HIP_TRY(hipGetLastError());
+        return ST_OK;
+    }
+    rc = ensure_sort_buffers(h, pairs->capacity);
+    if (rc) return rc;
+    CoopArgs ca;
+    ca.h = ha;
+    ca.kbuf[0] = h->keys_a;
+    ca.kbuf[1] = h->keys_b;
+    ca.vbuf[0] = h->vals_a;
+    ca.vbuf[1] = h->vals_b;
+    ca.block_counts = h->block_counts;
+    ca.kmax = h->kmax;
+    ca.barrier = h->barrier;
+    void *args[] = {&ca};
+    ST_HIP_TRY(hipLaunchCooperativeKernel((const void *)hist_coop_kernel, dim3(h->coop_grid),
+                                          dim3(kSortThreads), args, kLdsBytes, stream));
+    return ST_OK;
+}
+
+}  // namespace st
+
+using namespace st;
+
+extern "C" {
+
+// ---------------------------------------------------------------------------
+// Track / VolumeEdepPair
+// ---------------------------------------------------------------------------
+int st_track_alloc(st_track *t, int64_t capacity)
+{
+    if (!t || capacity < 0) return set_error(ST_EINVAL, "st_track_alloc: bad arguments");
+    std::memset(t, 0, sizeof *t);
+    // Track::AllocateEmptyTrack (Track.cu:112-118); one spare slot like the
+    // reference's n + 1 allocation (Track.cu:16).
+    const size_t bytes = (size_t)(capacity + 1) * sizeof(double);
+    double **arrs[4] = {&t->x, &t->y, &t->z, &t->edep};
+    for (double **p : arrs) {
+        hipError_t e = hipMalloc((void **)p, bytes);
+        if (e != hipSuccess) {
+            st_track_free(t);
+            return set_error(ST_ENOMEM, "st_track_alloc: hipMalloc(%zu) failed: %s", bytes,
+                             hipGetErrorString(e));
+        }
+    }
+    t->capacity = capacity;
+    t->n = 0;
+    return ST_OK;
+}
+
+int st_track_upload(st_track *t, const double *x, const double *y, const double *z,
+                    const double *edep, int64_t n)
+{
+    if (!t || n < 0 || (n > 0 && (!x || !y || !z || !edep)))
+        return set_error(ST_EINVAL, "st_track_upload: bad arguments");
+    if (!t->x || n > t->capacity) {
+        st_track_free(t);
+        int rc = st_track_alloc(t, n);
+        if (rc) return rc;
+    }
+    const size_t bytes = (size_t)n * sizeof(double);
+    if (bytes) {
+        ST_HIP_TRY(hipDeviceSynchronize());
+        ST_HIP_TRY(hipMemcpy(t->x, x, bytes, hipMemcpyHostToDevice));
+        ST_HIP_TRY(hipMemcpy(t->y, y, bytes, hipMemcpyHostToDevice));
+        ST_HIP_TRY(hipMemcpy(t->z, z, bytes, hipMemcpyHostToDevice));
+        ST_HIP_TRY(hipMemcpy(t->edep, edep, bytes, hipMemcpyHostToDevice));
+        ST_HIP_TRY(hipDeviceSynchronize());
+    }
+    t->n = n;
+    return ST_OK;
+}
+
+void st_track_free(st_track *t)
+{
+    if (!t) return;
+    (void)hipFree(t->x);
+    (void)hipFree(t->y);
+    (void)hipFree(t->z);
+    (void)hipFree(t->edep);
+    t->x = t->y = t->z = t->edep = nullptr;
+    t->n = t->capacity = 0;
+}
+
+// VolumeEdepPair::Allocate (VolumeEdepPair.cu:6-17): numElements starts at
+// the capacity and the arrays start zeroed.
+int st_pairs_alloc(st_pairs *p, int64_t capacity)
+{
+    if (!p || capacity < 0 || capacity > 0x7FFFFFFF)
+        return set_error(ST_EINVAL, "st_pairs_alloc: capacity must be in [0, 2^31)");
+    std::memset(p, 0, sizeof *p);
+    const size_t n = (size_t)(capacity > 0 ? capacity : 1);
+    hipError_t e1 = hipMalloc((void **)&p->volume, n * sizeof(uint64_t));
+    hipError_t e2 = hipMalloc((void **)&p->edep, n * sizeof(double));
+    hipError_t e3 = hipMalloc((void **)&p->num_elements, sizeof(int32_t));
+    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
+        st_pairs_free(p);
+        return set_error(ST_ENOMEM, "st_pairs_alloc: hipMalloc failed");
+    }
+    ST_HIP_TRY(hipMemset(p->volume, 0, n * sizeof(uint64_t)));
+    ST_HIP_TRY(hipMemset(p->edep, 0, n * sizeof(double)));
+    const int32_t c = (int32_t)capacity;
+    ST_HIP_TRY(hipMemcpy(p->num_elements, &c, sizeof c, hipMemcpyHostToDevice));
+    p->capacity = capacity;
+    return ST_OK;
+}
+
+void st_pairs_free(st_pairs *p)
+{
+    if (!p) return;
+    (void)hipFree(p->volume);
+    (void)hipFree(p->edep);
+    (void)hipFree(p->num_elements);
+    p->volume = nullptr;
+    p->edep = nullptr;
+    p->num_elements = nullptr;
+    p->capacity = 0;
+}
+
+// ---------------------------------------------------------------------------
+// VoxelConstrainedSphereMethod
+// ---------------------------------------------------------------------------
+int st_vcs_create(const st_geometry *g, int32_t shift_xy, int32_t shift_z,
+                  int32_t suggested_blocks, int32_t suggested_threads, st_vcs **out)
+{
+    if (!g || !out) return set_error(ST_EINVAL, "st_vcs_create: null argument");
+    *out = nullptr;
+    if (!(g->diameter > 0) || g->n_linear < 1)
+        return set_error(ST_EINVAL, "st_vcs_create: invalid geometry");
+    st_vcs *m = new (std::nothrow) st_vcs;
+    if (!m) return set_error(ST_ENOMEM, "st_vcs_create: out of host memory");
+    m->g = *g;
+    m->shift_xy = shift_xy ? 1 : 0;
+    m->shift_z = shift_z ? 1 : 0;
+    m->suggested_blocks = suggested_blocks;
+    m->suggested_threads = suggested_threads;
+    m->stream = default_stream();
+    hipError_t e = hipMalloc((void **)&m->ticket, sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMemset(m->ticket, 0, sizeof(unsigned long long));
+    if (e != hipSuccess) {
+        st_vcs_destroy(m);
+        return set_error(ST_EHIP, "st_vcs_create: %s", hipGetErrorString(e));
+    }
+    *out = m;
+    return ST_OK;
+}
+
+int st_vcs_set_stream(st_vcs *m, void *hip_stream)
+{
+    if (!m) return set_error(ST_EINVAL, "st_vcs_set_stream: null method");
+    m->stream = hip_stream ? (hipStream_t)hip_stream : default_stream();
+    return ST_OK;
+}
+
+int st_vcs_begin_track(st_vcs *m, st_track *track, uint64_t seed, uint32_t track_index,
+                       int32_t n_oversamples)
+{
+    if (!m || !track) return set_error(ST_EINVAL, "st_vcs_begin_track: null argument");
+    if (n_oversamples < 0) return set_error(ST_EINVAL, "st_vcs_begin_track: n_oversamples < 0");
+    if (track->n < 0 || track->n > ((int64_t)1 << 31) - 1)
+        return set_error(ST_ERANGE, "st_vcs_begin_track: track of %lld points (limit 2^31 - 1)",
+                         (long long)track->n);
+    if (track->n > 0 && (!track->x || !track->y || !track->z || !track->edep))
+        return set_error(ST_EINVAL, "st_vcs_begin_track: track has null arrays");
+    const int64_t n = track->n;
+    int rc = grow(&m->lineal, &m->lineal_capacity, std::max<int64_t>(n, 1));
+    if (rc) return rc;
+    int64_t shifts_cap = m->shifts_capacity;
+    rc = grow(&m->shifts, &shifts_cap, std::max<int64_t>(3 * (int64_t)n_oversamples, 3));
+    if (rc) return rc;
+    m->shifts_capacity = (int32_t)shifts_cap;
+    const int64_t tiles = std::max<int64_t>(1, (n + kScoreTile - 1) / kScoreTile);
+    if (tiles > m->status_capacity) {
+        rc = grow(&m->status, &m->status_capacity, tiles);
+        if (rc) return rc;
+        ST_HIP_TRY(hipMemsetAsync(m->status, 0, (size_t)tiles * sizeof(unsigned long long), m->stream));
+    }
+    // ShiftTrackZOrigin, once per track (VoxelConstrainedSphereMethod.cu:45)
+    if (n > 0) {
+        zshift_kernel<<<grid_for(n, 256, 4096), 256, 0, m->stream>>>(track->z, n, m->g.half_length);
+        lineal_kernel<<<grid_for(n, 256, 4096), 256, 0, m->stream>>>(track->edep, m->lineal, n,
+                                                                       (2. / 3.) * m->g.diameter);
+    }
+    if (n_oversamples > 0)
+        shift_table_kernel<<<grid_for(n_oversamples, 256), 256, 0, m->stream>>>(
+            m->shifts, n_oversamples, seed, track_index, m->shift_xy, m->shift_z, m->g.length,
+            m->g.half_length);
+    ST_HIP_TRY(hipGetLastError());
+    m->n = n;
+    m->n_oversamples = n_oversamples;
+    m->oversample = 0;
+    m->in_track = true;
+    return ST_OK;
+}
+
+int st_vcs_process_oversample(st_vcs *m, const st_track *track, st_pairs *out)
+{
+    if (!m || !track || !out) return set_error(ST_EINVAL, "st_vcs_process_oversample: null argument");
+    if (!m->in_track) return set_error(ST_ESTATE, "st_vcs_process_oversample before st_vcs_begin_track");
+    if (m->oversample >= m->n_oversamples)
+        return set_error(ST_ESTATE, "st_vcs_process_oversample: all %d oversamples already processed",
+                         m->n_oversamples);
+    if (track->n != m->n)
+        return set_error(ST_EINVAL, "st_vcs_process_oversample: track has %lld points, begun with %lld",
+                         (long long)track->n, (long long)m->n);
+    if (out->capacity < m->n || !out->volume || !out->edep || !out->num_elements)
+        return set_error(ST_EINVAL, "st_vcs_process_oversample: pairs capacity %lld < %lld points",
+                         (long long)out->capacity, (long long)m->n);
+    const int64_t n = m->n;
+    const int tiles = (int)std::max<int64_t>(1, (n + kScoreTile - 1) / kScoreTile);
+    ScoreArgs a;
+    a.x = track->x;
+    a.y = track->y;
+    a.z = track->z;
+    a.lin = m->lineal;
+    a.n = n;
+    a.H = m->g.half_length;
+    a.D = m->g.diameter;
+    a.invD = 1.0 / m->g.diameter;
+    a.gso = m->g.gso;
+    a.r2 = m->g.radius * m->g.radius;
+    a.N = (uint64_t)m->g.n_linear;
+    a.shifts = m->shifts;
+    a.os = m->oversample;
+    a.out_key = out->volume;
+    a.out_y = out->edep;
+    a.out_count = out->num_elements;
+    a.status = m->status;
+    a.ticket = m->ticket;
+    a.ticket_base = m->ticket_base;
+    a.epoch = ++m->epoch;
+    if ((a.epoch & 0x3FFFFFFFu) == 0) a.epoch = ++m->epoch;
+    a.num_tiles = tiles;
+    score_kernel<<<tiles, kScoreBlock, 0, m->stream>>>(a);
+    ST_HIP_TRY(hipGetLastError());
+    m->ticket_base += (uint64_t)tiles;
+    m->oversample++;
+    return ST_OK;
+}
+
+int st_vcs_end_track(st_vcs *m)
+{
+    if (!m) return set_error(ST_EINVAL, "st_vcs_end_track: null method");
+    ST_HIP_TRY(hipStreamSynchronize(m->stream));
+    m->in_track = false;
+    return ST_OK;
+}
+
+void st_vcs_destroy(st_vcs *m)
+{
+    if (!m) return;
+    if (m->stream) (void)hipStreamSynchronize(m->stream);
+    (void)hipFree(m->lineal);
+    (void)hipFree(m->shifts);
+    (void)hipFree(m->status);
+    (void)hipFree(m->ticket);
+    delete m;
+}
+
+// ---------------------------------------------------------------------------
+// Histogram
+// ---------------------------------------------------------------------------
+int st_hist_create(int32_t type, float lo, float hi, int32_t nbins, st_hist **out)
+{
+    if (!out) return set_error(ST_EINVAL, "st_hist_create: null output");
+    *out = nullptr;
+    if (type != ST_HIST_LIN && type != ST_HIST_LOG)
+        return set_error(ST_EINVAL, "Unknown histogram type %d: available types are lin (0) and log (1)",
+                         type);
+    if (nbins < 1 || nbins > (1 << 24))
+        return set_error(ST_EINVAL, "st_hist_create: nbins must be in [1, 2^24] (got %d)", nbins);
+    st_hist *h = new (std::nothrow) st_hist;
+    if (!h) return set_error(ST_ENOMEM, "st_hist_create: out of host memory");
+    h->type = type;
+    h->nbins = nbins;
+    h->lo = lo;
+    h->hi = hi;
+    h->edges_host.resize((size_t)nbins + 1);
+    make_edges(type, lo, hi, nbins, h->edges_host.data());
+    h->guess.type = type;
+    h->guess.nbins = nbins;
+    if (type == ST_HIST_LIN) {
+        const float wf = (hi - lo) / (float)nbins;
+        h->guess.origin = (double)lo;
+        h->guess.inv_width = 1.0 / (double)wf;
+    } else {
+        const float bo = (float)std::log10((double)lo);
+        const float to = (float)std::log10((double)hi);
+        const float wf = (to - bo) / (float)nbins;
+        h->guess.origin = (double)bo;
+        h->guess.inv_width = 1.0 / (double)wf;
+    }
+    if (!std::isfinite(h->guess.inv_width)) h->guess.inv_width = 0.0;
+    if (!std::isfinite(h->guess.origin)) h->guess.origin = 0.0;
+    h->stream = default_stream();
+    hipError_t e = hipMalloc((void **)&h->edges_dev, ((size_t)nbins + 1) * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc((void **)&h->bins_dev, (size_t)nbins * sizeof(uint64_t));
+    if (e == hipSuccess)
+        e = hipMemcpy(h->edges_dev, h->edges_host.data(), ((size_t)nbins + 1) * sizeof(double),
+                      hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(h->bins_dev, 0, (size_t)nbins * sizeof(uint64_t));
+    if (e != hipSuccess) {
+        st_hist_destroy(h);
+        return set_error(ST_EHIP, "st_hist_create: %s", hipGetErrorString(e));
+    }
+    h->host_counts.resize(nbins);
+    *out = h;
+    return ST_OK;
+}
+
+int st_hist_set_stream(st_hist *h, void *hip_stream)
+{
+    if (!h) return set_error(ST_EINVAL, "st_hist_set_stream: null histogram");
+    h->stream = hip_stream ? (hipStream_t)hip_stream : default_stream();
+    return ST_OK;
+}
+
+int st_hist_nbins(const st_hist *h, int32_t *nbins)
+{
+    if (!h || !nbins) return set_error(ST_EINVAL, "st_hist_nbins: null argument");
+    *nbins = h->nbins;
+    return ST_OK;
+}
+
+int st_hist_edges(const st_hist *h, double *edges)
+{
+    if (!h || !edges) return set_error(ST_EINVAL, "st_hist_edges: null argument");
+    std::memcpy(edges, h->edges_host.data(), h->edges_host.size() * sizeof(double));
+    return ST_OK;
+}
+
+// Histogram::AllocateTrackProcess (Histogram.cu:102-112): size the sort
+// buffers for the pairs' capacity.  Buffers are kept and only grown, instead
+// of being allocated and freed per track.
+int st_hist_begin_track(st_hist *h, const st_pairs *pairs)
+{
+    if (!h || !pairs) return set_error(ST_EINVAL, "st_hist_begin_track: null argument");
+    if (pairs->capacity < 0 || pairs->capacity > 0x7FFFFFFF)
+        return set_error(ST_EINVAL, "st_hist_begin_track: bad pairs capacity");
+    h->capacity = pairs->capacity;
+    if (h->capacity > kLdsCap) {
+        int rc = ensure_sort_buffers(h, h->capacity);
+        if (rc) return rc;
+    }
+    // the per-track counters start at zero (Histogram.cu:139-140)
+    zero_u64_kernel<<<grid_for(h->nbins, 256, 1024), 256, 0, h->stream>>>(
+        (unsigned long long *)h->bins_dev, h->nbins);
+    ST_HIP_TRY(hipGetLastError());
+    h->in_track = true;
+    return ST_OK;
+}
+
+// Histogram::SortReduceAndAddToHistogram (Histogram.cu:114-123,147-171).
+int st_hist_add(st_hist *h, const st_pairs *pairs)
+{
+    if (!h || !pairs) return set_error(ST_EINVAL, "st_hist_add: null argument");
+    if (!h->in_track) return set_error(ST_ESTATE, "st_hist_add before st_hist_begin_track");
+    if (pairs->capacity > h->capacity)
+        return set_error(ST_EINVAL, "st_hist_add: pairs capacity %lld above the begun %lld",
+                         (long long)pairs->capacity, (long long)h->capacity);
+    return hist_add_pairs(h, pairs, h->stream);
+}
+
+// Histogram::FreeTrackProcess -> AccumulateCPUHistogram (Histogram.cu:125-141,
+// 177-186): host_bins[i + 1] += count[i].
+int st_hist_end_track(st_hist *h, double *host_bins)
+{
+    if (!h) return set_error(ST_EINVAL, "st_hist_end_track: null histogram");
+    if (!h->in_track) return set_error(ST_ESTATE, "st_hist_end_track before st_hist_begin_track");
+    ST_HIP_TRY(hipMemcpyAsync(h->host_counts.data(), h->bins_dev, (size_t)h->nbins * sizeof(uint64_t),
+                              hipMemcpyDeviceToHost, h->stream));
+    ST_HIP_TRY(hipStreamSynchronize(h->stream));
+    if (host_bins)
+        for (int i = 0; i < h->nbins; ++i) host_bins[i + 1] += (double)h->host_counts[i];
+    zero_u64_kernel<<<grid_for(h->nbins, 256, 1024), 256, 0, h->stream>>>(
+        (unsigned long long *)h->bins_dev, h->nbins);
+    ST_HIP_TRY(hipGetLastError());
+    h->in_track = false;
+    return ST_OK;
+}
+
+int st_hist_fetch(st_hist *h, uint64_t *counts, int32_t reset)
+{
+    if (!h || !counts) return set_error(ST_EINVAL, "st_hist_fetch: null argument");
+    ST_HIP_TRY(hipMemcpyAsync(counts, h->bins_dev, (size_t)h->nbins * sizeof(uint64_t),
+                              hipMemcpyDeviceToHost, h->stream));
+    ST_HIP_TRY(hipStreamSynchronize(h->stream));
+    if (reset) {
+        zero_u64_kernel<<<grid_for(h->nbins, 256, 1024), 256, 0, h->stream>>>(
+            (unsigned long long *)h->bins_dev, h->nbins);
+        ST_HIP_TRY(hipGetLastError());
+        ST_HIP_TRY(hipStreamSynchronize(h->stream));
+    }
+    return ST_OK;
+}
+
+void st_hist_destroy(st_hist *h)
+{
+    if (!h) return;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    (void)hipFree(h->edges_dev);
+    (void)hipFree(h->bins_dev);
+    (void)hipFree(h->keys_a);
+    (void)hipFree(h->keys_b);
+    (void)hipFree(h->vals_a);
+    (void)hipFree(h->vals_b);
+    (void)hipFree(h->block_counts);
+    (void)hipFree(h->kmax);
+    (void)hipFree(h->barrier);
+    delete h;
+}
+
+}  // extern "C"

@@ -1,0 +1,93 @@
+// st_internal.hpp -- internal declarations shared by the translation units of
+// libsupertrack_amd.
+#pragma once
+
+#include <functional>
+#include <vector>
+
+#include "st_common.hpp"
+
+namespace st {
+
+int device_cu_count();
+int host_threads();
+void parallel_for(int64_t n, const std::function<void(int64_t)> &fn);
+void make_geometry(double half_length_ini, double diameter_ini, st_geometry *g);
+void make_edges(int type, float lo, float hi, int nbins, double *edges);
+
+// Parameters of the closed-form bin guess handed to kernels with the edges.
+struct BinGuess {
+    int type;        // ST_HIST_LIN / ST_HIST_LOG
+    int nbins;
+    double origin;   // lin: edges[0]; log: float log10(lo) widened
+    double inv_width;
+};
+
+__host__ __device__ inline int bin_guess(const BinGuess &g, double v)
+{
+    double t;
+    if (g.type == ST_HIST_LIN) {
+        t = (v - g.origin) * g.inv_width;
+    } else {
+        t = (v > 0.0) ? (log10(v) - g.origin) * g.inv_width : -1.0;
+    }
+    if (!(t > -1.0)) return 0;            // also catches NaN
+    if (t > 2147483000.0) return g.nbins - 1;
+    return (int)t;
+}
+
+}  // namespace st
+
+struct st_hist;
+struct st_vcs;
+namespace st {
+// Sorts, reduces and bins pairs into h's device counters on `stream`.
+int hist_add_pairs(st_hist *h, const st_pairs *pairs, hipStream_t stream);
+}  // namespace st
+
+// Histogram object (Histogram.cuh:19-78): bin edges, device counters and the
+// per-track sort buffers.
+struct st_hist {
+    int type = ST_HIST_LIN;
+    int nbins = 0;
+    float lo = 0, hi = 0;
+    std::vector<double> edges_host;
+    st::BinGuess guess{};
+    double *edges_dev = nullptr;     // nbins + 1
+    uint64_t *bins_dev = nullptr;    // nbins counters
+    hipStream_t stream = nullptr;
+    int64_t capacity = 0;            // pairs capacity of the current track
+    bool in_track = false;
+    // cooperative sort buffers (pairs capacities above the LDS path's limit)
+    int64_t sort_capacity = 0;
+    uint64_t *keys_a = nullptr, *keys_b = nullptr;
+    uint32_t *vals_a = nullptr, *vals_b = nullptr;
+    uint32_t *block_counts = nullptr;     // [256 digits][coop_grid]
+    unsigned long long *kmax = nullptr;
+    unsigned int *barrier = nullptr;
+    int coop_grid = 0;
+    std::vector<uint64_t> host_counts;
+};
+
+// VoxelConstrainedSphereMethod object (VoxelConstrainedSphereMethod.hh:43-75).
+struct st_vcs {
+    st_geometry g{};
+    int shift_xy = 1, shift_z = 0;
+    int suggested_blocks = 0, suggested_threads = 0;
+    hipStream_t stream = nullptr;
+    // per-track state (AllocateTrackProcess .. FreeTrackProcess)
+    bool in_track = false;
+    int64_t n = 0;
+    int32_t n_oversamples = 0;
+    int32_t oversample = 0;
+    int64_t lineal_capacity = 0;
+    double *lineal = nullptr;            // e / ((2/3) D) per point, computed once per track
+    double *shifts = nullptr;            // 3 * n_oversamples
+    int32_t shifts_capacity = 0;
+    // decoupled look-back state of the fused scoring kernel
+    unsigned long long *status = nullptr;
+    int64_t status_capacity = 0;
+    unsigned long long *ticket = nullptr;
+    uint64_t ticket_base = 0;
+    uint32_t epoch = 0;
+};
