@@ -16,7 +16,9 @@ import time
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
+HOST = os.path.join(HERE, "host")
 LIB = os.path.join(HERE, "libsupertrack_amd.so")
+CLI = os.path.join(HERE, "supertrack")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SUPERTRACK_ARCH", "gfx950")
 
@@ -40,6 +42,9 @@ def sources():
     for name in sorted(os.listdir(CSRC)):
         if name.endswith(".hip") or name.endswith(".cpp"):
             out.append(os.path.join(CSRC, name))
+    for name in sorted(os.listdir(HOST)):
+        if name.endswith(".cc") and name != "supertrack_main.cc":
+            out.append(os.path.join(HOST, name))
     return out
 
 
@@ -61,7 +66,8 @@ def build_library(jobs: int = 8, verbose: bool = False) -> str:
         obj = os.path.join(objdir, os.path.basename(src) + ".o")
         objs.append(obj)
         lang = ["-x", "hip"] if src.endswith(".hip") else []
-        cmd = [HIPCC] + COMMON + DEVICE + lang + ["-I", os.path.join(ROOT, "include"), "-c", src, "-o", obj]
+        cmd = ([HIPCC] + COMMON + DEVICE + lang +
+               ["-I", os.path.join(ROOT, "include"), "-I", HOST, "-c", src, "-o", obj])
         procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
         while len([p for _, p in procs if p.poll() is None]) >= jobs:
             time.sleep(0.05)
@@ -79,6 +85,12 @@ def build_library(jobs: int = 8, verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs + ["-lpthread"])
     os.replace(tmp, LIB)
+    # command line driver linked against the library (found next to it)
+    tmp = CLI + ".tmp"
+    _run([HIPCC, "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), "-I", HOST,
+          os.path.join(HOST, "supertrack_main.cc"), "-o", tmp, "-L", HERE, "-lsupertrack_amd",
+          "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, CLI)
     return LIB
 
 

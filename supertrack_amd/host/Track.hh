@@ -1,0 +1,22 @@
+// Track.hh -- device SoA of one track (Track.cuh:7-23).
+#pragma once
+
+#include "supertrack_amd.h"
+
+class ThreadTask;
+
+class Track {
+public:
+    Track() = default;
+    void Free();
+    // Reads the task's entries from its track file and copies them to the GPU.
+    void AllocateAndLoadTrack(const ThreadTask &task);
+    void AllocateEmptyTrack(int nVals);
+
+    double *x = nullptr;
+    double *y = nullptr;
+    double *z = nullptr;
+    double *edep = nullptr;
+
+    st_track c{};   // the C ABI view of the same buffers
+};
