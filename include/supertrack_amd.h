@@ -177,7 +177,7 @@ typedef struct {
     int32_t use_graph;            /* capture the launch sequence once, replay */
     int32_t autotune;             /* pick launch parameters by timing          */
     int32_t block_threads;        /* 0 = default / autotuned                   */
-    int32_t table_slots;          /* LDS hash slots per workgroup (power of 2) */
+    int32_t table_slots;          /* LDS hash slots per workgroup: 2048 or 4096 */
     int32_t blocks_per_cu;        /* persistent workgroups per CU              */
     int32_t pad;
 } st_batch_opts;
@@ -191,6 +191,9 @@ int st_batch_set_stream(st_batch *b, void *hip_stream);
  * counters of each histogram (read them with st_hist_fetch). */
 int st_batch_run(st_batch *b);
 int st_batch_sync(st_batch *b);
+/* Device time of the last st_batch_run in ms: ms2[0] the fused kernel
+ * (with its reset), ms2[1] the exact per-segment path. */
+int st_batch_times(st_batch *b, float *ms2);
 /* Launch parameters in use: block_threads, table_slots, blocks_per_cu. */
 int st_batch_params(const st_batch *b, int32_t *params3);
 /* Test hook: record (segment, key, sum) of every scored sphere of the next

@@ -356,6 +356,12 @@ class BatchRun:
     def sync(self):
         call("st_batch_sync", self.h)
 
+    def times(self):
+        """(fused kernel ms, exact path ms) of the last run."""
+        t = (C.c_float * 2)()
+        call("st_batch_times", self.h, t)
+        return float(t[0]), float(t[1])
+
     def params(self):
         p = np.zeros(3, np.int32)
         call("st_batch_params", self.h, p.ctypes.data_as(_capi.c_int32_p))

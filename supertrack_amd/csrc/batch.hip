@@ -47,7 +47,7 @@ constexpr int kBlk = 256;                   // points per library block (== chun
 constexpr unsigned long long kEmpty = ~0ull;
 constexpr int kMaxGeoms = 16;
 constexpr int kMaxHists = 2;
-constexpr int64_t kMaxWindow = 512;         // fast-path bound on spheres in a 4-layer window
+constexpr int64_t kMaxWindow = 768;         // fast-path bound on spheres in a 2-layer window
 constexpr int64_t kMaxFastPoints = (int64_t)1 << 22;
 constexpr uint32_t kFlagExact = 1u;
 
@@ -102,7 +102,8 @@ struct BatchArgs {
     // histograms
     HistDev hist[kMaxHists];
     int n_hists;
-    int hist_lds_words;
+    int hist_lds_words;     // 0: bins go straight to the device counters
+
     // recording (test hook)
     int64_t rec_cap;
     unsigned long long *rec_count;
@@ -221,7 +222,11 @@ __device__ int wave_lower_bound(int n, Pred pred)
 template <int MODE, int SLOTS>
 __global__ __launch_bounds__(kBT) void batch_kernel(BatchArgs a)
 {
-    constexpr int kLog2Slots = (SLOTS == 1024) ? 10 : (SLOTS == 2048) ? 11 : 12;
+    static_assert(SLOTS == 2048 || SLOTS == 4096, "table sizes");
+    // occupancy never exceeds max(half table, survivors) + one chunk
+    static_assert((SLOTS / 2 > kMaxWindow ? SLOTS / 2 : kMaxWindow) + kBlk <= SLOTS * 9 / 10,
+                  "the table must hold the survivors of a flush plus a chunk");
+    constexpr int kLog2Slots = (SLOTS == 2048) ? 11 : 12;
     constexpr int kPer = SLOTS / kBT;            // table slots scanned per thread
     constexpr int kWords = Acc<MODE>::words;
     extern __shared__ unsigned long long lds[];
@@ -256,7 +261,10 @@ __global__ __launch_bounds__(kBT) void batch_kernel(BatchArgs a)
             if (h < a.n_hists) {
                 const HistDev &hd = a.hist[h];
                 const int b = bin_of(hd.edges, hd.guess.nbins, v, bin_guess(hd.guess, v));
-                if (b >= 0) atomicAdd(&lhist[hd.lds_offset + b], 1u);
+                if (b >= 0) {
+                    if (a.hist_lds_words) atomicAdd(&lhist[hd.lds_offset + b], 1u);
+                    else atomicAdd(&hd.bins[b], 1ull);
+                }
             }
         }
         if (a.rec_cap > 0) {
@@ -424,17 +432,18 @@ __global__ __launch_bounds__(kBT) void batch_kernel(BatchArgs a)
             __syncthreads();
             const int64_t next = c + kBlk;
             if (s_occ >= SLOTS / 2 && next < p_end) {
-                // every later point lies in layer >= rint(q) - 1 of the next point
+                // every later point lies in a layer >= the next point's exact
+                // layer index (z order), and a key's layer (key / N^2) is never
+                // below its point's layer: entries with smaller keys are complete
                 const double zs = (a.z64[next] + g.H) + sh.z;
-                const double q = (zs - g.gso) * g.invD;
-                double lb = rint(q) - 1.0;
+                double lb = nearest_index(zs - g.gso, g.D, g.invD);
                 if (!(lb > 0.0)) lb = 0.0;
                 const unsigned long long bound = (unsigned long long)lb * g.N2;
                 flush(bound, seg_id);
             }
         }
         flush(kEmpty, seg_id);
-        if (s_binned > 0x40000000ull) {
+        if (a.hist_lds_words && s_binned > 0x40000000ull) {
             // keep the 32-bit LDS counters far from overflow
 #pragma unroll
             for (int h = 0; h < kMaxHists; ++h) {
@@ -454,6 +463,7 @@ __global__ __launch_bounds__(kBT) void batch_kernel(BatchArgs a)
         }
     }
     __syncthreads();
+    if (a.hist_lds_words == 0) return;
 #pragma unroll
     for (int h = 0; h < kMaxHists; ++h) {
         if (h >= a.n_hists) continue;
@@ -482,14 +492,8 @@ const void *kernel_ptr()
 
 const void *pick_kernel(int mode, int slots)
 {
-    if (mode == ST_F64) {
-        if (slots == 1024) return kernel_ptr<ST_F64, 1024>();
-        if (slots == 2048) return kernel_ptr<ST_F64, 2048>();
-        return kernel_ptr<ST_F64, 4096>();
-    }
-    if (slots == 1024) return kernel_ptr<ST_F32, 1024>();
-    if (slots == 2048) return kernel_ptr<ST_F32, 2048>();
-    return kernel_ptr<ST_F32, 4096>();
+    if (mode == ST_F64) return slots == 2048 ? kernel_ptr<ST_F64, 2048>() : kernel_ptr<ST_F64, 4096>();
+    return slots == 2048 ? kernel_ptr<ST_F32, 2048>() : kernel_ptr<ST_F32, 4096>();
 }
 
 size_t lds_bytes(int mode, int slots, int hist_words)
@@ -575,6 +579,7 @@ struct st_batch {
     hipStream_t stream = nullptr;
     int threads = kBT, slots = 2048, blocks_per_cu = 4, grid = 0;
     int hist_words = 0;
+    bool lds_hist = true;     // histograms privatised in LDS (else device atomics)
     size_t lds = 0;
     int32_t *order_dev = nullptr;
     unsigned long long *ticket = nullptr;
@@ -587,6 +592,8 @@ struct st_batch {
     bool graph_ready = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    // timing of the last run: fused kernel (graph) and exact path
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     // exact path
     std::vector<st_vcs *> methods;    // one per geometry
     st_track scratch{};
@@ -630,7 +637,7 @@ BatchArgs make_args(st_batch *b)
         a.hist[h].lds_offset = off;
         off += b->hists[h]->nbins;
     }
-    a.hist_lds_words = off;
+    a.hist_lds_words = b->lds_hist ? off : 0;
     a.rec_cap = b->rec_cap;
     a.rec_count = b->rec_count;
     a.rec_seg = b->rec_seg;
@@ -644,7 +651,7 @@ int launch_fast(st_batch *b, int slots, int grid)
     BatchArgs a = make_args(b);
     batch_reset_kernel<<<1, 64, 0, b->stream>>>(b->ticket, b->error, b->rec_cap > 0 ? b->rec_count : nullptr);
     if (a.n_seg == 0) return ST_OK;
-    const size_t lds = lds_bytes(b->lib->precision, slots, b->hist_words);
+    const size_t lds = lds_bytes(b->lib->precision, slots, b->lds_hist ? b->hist_words : 0);
     const void *k = pick_kernel(b->lib->precision, slots);
     void *args[] = {&a};
     ST_HIP_TRY(hipLaunchKernel(k, dim3(grid), dim3(kBT), args, lds, b->stream));
@@ -653,9 +660,12 @@ int launch_fast(st_batch *b, int slots, int grid)
 
 int configure(st_batch *b, int slots, int bpc)
 {
-    const size_t lds = lds_bytes(b->lib->precision, slots, b->hist_words);
+    // privatise the histograms in LDS when they fit next to the table at two
+    // workgroups per CU; otherwise count straight into the device bins
+    b->lds_hist = lds_bytes(b->lib->precision, slots, b->hist_words) <= 80 * 1024;
+    const size_t lds = lds_bytes(b->lib->precision, slots, b->lds_hist ? b->hist_words : 0);
     if (lds > 160 * 1024)
-        return set_error(ST_ERANGE, "batched kernel needs %zu B of LDS (limit 160 KiB): too many bins", lds);
+        return set_error(ST_ERANGE, "batched kernel needs %zu B of LDS (limit 160 KiB)", lds);
     const void *k = pick_kernel(b->lib->precision, slots);
     ST_HIP_TRY(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     int per_cu = 0;
@@ -812,8 +822,11 @@ int st_library_create(const double *x, const double *y, const double *z, const d
         const double ylim = (precision == ST_F64) ? 2.7e11 : 1.0e9;
         bool fast = finite && n <= kMaxFastPoints && ysum < ylim && emax / denom < ylim;
         if (fast) {
-            const double w = 4.0 * g.diameter * (1.0 + 1e-12) + 1e-9;
-            const int64_t W = window_bound(sx.data() + o, sy.data() + o, sz.data() + o, n, g.diameter, w, 5);
+            // entries kept by a flush come from the next point's layer (or, for
+            // keys with an index equal to N, the layer below): a z-window two
+            // diameters wide, touching at most three layers
+            const double w = 2.0 * g.diameter * (1.0 + 1e-12) + 1e-9;
+            const int64_t W = window_bound(sx.data() + o, sy.data() + o, sz.data() + o, n, g.diameter, w, 3);
             fast = W <= kMaxWindow;
         }
         td.flags = fast ? 0u : kFlagExact;
@@ -960,15 +973,17 @@ int st_batch_create(st_library *lib, const st_batch_opts *opts, st_hist *const *
         st_vcs_set_stream(m, b->stream);
         b->methods.push_back(m);
     }
+    // 2048 slots hold the largest survivor set (kMaxWindow) plus a half table
+    // plus a chunk, so every fast-path track fits for every choice
     int slots = opts->table_slots ? opts->table_slots : 2048;
-    if (slots != 1024 && slots != 2048 && slots != 4096)
-        return fail(set_error(ST_EINVAL, "st_batch_create: table_slots must be 1024, 2048 or 4096"));
+    if (slots != 2048 && slots != 4096)
+        return fail(set_error(ST_EINVAL, "st_batch_create: table_slots must be 2048 or 4096"));
     int bpc = opts->blocks_per_cu ? opts->blocks_per_cu : 4;
     rc = configure(b, slots, bpc);
     if (rc) return fail(rc);
     if (opts->autotune && !lib->fast_order.empty() && opts->n_oversamples > 0) {
         // time each candidate on one full pass; keep the fastest
-        const int cand_slots[] = {1024, 2048, 4096};
+        const int cand_slots[] = {2048, 4096};
         const int cand_bpc[] = {2, 3, 4, 6, 8};
         hipEvent_t e0, e1;
         ST_HIP_TRY(hipEventCreate(&e0));
@@ -1032,6 +1047,9 @@ int st_batch_run(st_batch *b)
 {
     if (!b) return set_error(ST_EINVAL, "st_batch_run: null batch");
     if (b->opts.n_oversamples == 0) return ST_OK;
+    if (!b->ev[0])
+        for (hipEvent_t &e : b->ev) ST_HIP_TRY(hipEventCreate(&e));
+    ST_HIP_TRY(hipEventRecord(b->ev[0], b->stream));
     if (b->opts.use_graph && b->rec_cap == 0) {
         if (!b->graph_ready) {
             ST_HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
@@ -1049,7 +1067,21 @@ int st_batch_run(st_batch *b)
         int rc = launch_fast(b, b->slots, b->grid);
         if (rc) return rc;
     }
-    return run_exact(b);
+    ST_HIP_TRY(hipEventRecord(b->ev[1], b->stream));
+    int rc = run_exact(b);
+    if (rc) return rc;
+    ST_HIP_TRY(hipEventRecord(b->ev[2], b->stream));
+    return ST_OK;
+}
+
+int st_batch_times(st_batch *b, float *ms2)
+{
+    if (!b || !ms2) return set_error(ST_EINVAL, "st_batch_times: null argument");
+    if (!b->ev[0]) return set_error(ST_ESTATE, "st_batch_times before st_batch_run");
+    ST_HIP_TRY(hipEventSynchronize(b->ev[2]));
+    ST_HIP_TRY(hipEventElapsedTime(&ms2[0], b->ev[0], b->ev[1]));
+    ST_HIP_TRY(hipEventElapsedTime(&ms2[1], b->ev[1], b->ev[2]));
+    return ST_OK;
 }
 
 int st_batch_sync(st_batch *b)
@@ -1111,6 +1143,8 @@ void st_batch_destroy(st_batch *b)
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     for (st_vcs *m : b->methods) st_vcs_destroy(m);
+    for (hipEvent_t e : b->ev)
+        if (e) (void)hipEventDestroy(e);
     st_track_free(&b->scratch);
     st_pairs_free(&b->pairs);
     void *ptrs[] = {b->order_dev, b->ticket, b->error, b->rec_count, b->rec_seg, b->rec_key, b->rec_sum};

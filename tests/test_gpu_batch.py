@@ -147,20 +147,24 @@ def test_batch_mixed_geometries_two_histograms():
 
 
 def test_batch_exact_path_tracks():
-    # a NaN point and a dense A5-like track force the exact path; bins stay exact
+    # a NaN point and an A3-like plane (3000 spheres per layer) force the
+    # exact path; an A5-like track (1000 coincident points per sphere) stays
+    # on the fused kernel; bins equal the oracle's
     w = W.workload("cfg2")
     x, y, z, e, off, _ = W.make_library(w, n_tracks=4)
     x = x.copy()
     x[off[1] + 5] = np.nan
+    px, py, pz, pe = F.a3()
+    keep = pz < pz.min() + 20e3               # the first 20 layers of the plane: 60000 points
     ax, ay, az, ae = F.a5_track()
     sel = slice(0, 200000)
-    xs = np.concatenate([x, ax[sel]]); ys = np.concatenate([y, ay[sel]])
-    zs = np.concatenate([z, az[sel]]); es = np.concatenate([e, ae[sel]])
-    off2 = np.concatenate([off, [off[-1] + 200000]]).astype(np.int64)
+    xs = np.concatenate([x, px[keep], ax[sel]]); ys = np.concatenate([y, py[keep], ay[sel]])
+    zs = np.concatenate([z, pz[keep], az[sel]]); es = np.concatenate([e, pe[keep], ae[sel]])
+    off2 = np.concatenate([off, [off[-1] + keep.sum(), off[-1] + keep.sum() + 200000]]).astype(np.int64)
     geoms = [(1.5e6, 1e3)]
     counts, _, info, edges = gpu_run(xs, ys, zs, es, off2, geoms, None, 3, 11, True, False, st.ST_F64,
                                      [("lin", 30000, 1e-1, 1e4)])
-    assert info["exact_tracks"] >= 2
+    assert info["exact_tracks"] == 2 and info["fast_tracks"] == 4
     bins, _, _ = po.run(xs, ys, zs, es, off2, [po.geometry(*geoms[0])], None, 3, 11, True, False, [edges[0]])
     np.testing.assert_array_equal(counts[0], bins[0])
 
