@@ -102,8 +102,10 @@ typedef struct {
     double *edep;            /* lineal energy per pair (keV/um), device    */
     int32_t *num_elements;   /* device counter, written by process_oversample */
     int64_t capacity;
+    uint64_t key_bound;      /* largest key the producer can write; 0 = unknown
+                                (set by st_vcs_process_oversample; the histogram
+                                sorts only the bits this needs) */
 } st_pairs;
-/* The histogram sorts only as many key bits as the largest key present. */
 
 int st_pairs_alloc(st_pairs *p, int64_t capacity);
 void st_pairs_free(st_pairs *p);

@@ -35,7 +35,7 @@ class Track(C.Structure):
 
 class Pairs(C.Structure):
     _fields_ = [("volume", C.c_void_p), ("edep", C.c_void_p), ("num_elements", C.c_void_p),
-                ("capacity", C.c_int64)]
+                ("capacity", C.c_int64), ("key_bound", C.c_uint64)]
 
 
 class LibraryInfo(C.Structure):

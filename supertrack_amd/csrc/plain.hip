@@ -225,9 +225,8 @@ __global__ __launch_bounds__(kScoreBlock) void score_kernel(ScoreArgs a)
 // Histogram: stable LSD radix sort -> left-to-right reduce-by-key -> bin
 // ---------------------------------------------------------------------------
 // Small inputs (m <= kLdsCap pairs) are sorted by one 1024-thread workgroup in
-// LDS with 4-bit digits; larger inputs by all workgroups of a cooperative
-// launch with 8-bit digits and grid barriers between phases.  Both sort only
-// as many bits as the largest key present has.
+// LDS with 4-bit digits; larger inputs by the onesweep kernels below with
+// 8-bit digits.  Only as many bits are sorted as the keys can have.
 constexpr int kSortThreads = 1024;
 constexpr int kSortWaves = kSortThreads / 64;
 constexpr int kLdsCap = 6144;
@@ -235,9 +234,6 @@ constexpr int kDigitBits = 4;
 constexpr int kDigits = 1 << kDigitBits;
 constexpr size_t kLdsBytes = (size_t)kLdsCap * (2 * sizeof(uint64_t) + 2 * sizeof(uint16_t)) +
                              (size_t)kDigits * kSortThreads * sizeof(uint16_t);
-constexpr int kGBits = 8;
-constexpr int kGDigits = 1 << kGBits;
-
 struct HistArgs {
     const uint64_t *keys;
     const double *y;
@@ -368,185 +364,216 @@ __global__ __launch_bounds__(kSortThreads) void hist_lds_kernel(HistArgs a)
     lds_sort_reduce_bin(a, m, smem);
 }
 
-// Grid barrier for a cooperative launch (all workgroups co-resident).
-// bar[0] counts arrivals, bar[1] is the generation.
-__device__ __forceinline__ void grid_barrier(unsigned int *bar, unsigned int nblocks)
+// ---------------------------------------------------------------------------
+// Histogram, m > kLdsCap: onesweep LSD radix sort over global buffers
+// ---------------------------------------------------------------------------
+// os_digits_kernel  all passes' digit counts of all pairs (one read)
+// os_pass_kernel    per 8-bit pass: each 2048-element tile ranks its digits
+//                   stably (rows of 256 in order, 8 wave ballots per row),
+//                   takes per-digit offsets from earlier tiles by a decoupled
+//                   look-back (status words tagged with an epoch, never
+//                   cleared) and scatters
+// os_reduce_kernel  left-to-right sums of each run, exact binning; clears the
+//                   digit counts for the next call
+constexpr int kGBits = 8;
+constexpr int kGDigits = 1 << kGBits;
+constexpr int kMaxPasses = 8;
+constexpr int kOsThreads = 256;
+constexpr int kOsWaves = kOsThreads / 64;
+constexpr int kOsRows = 8;
+constexpr int kOsTile = kOsThreads * kOsRows;
+
+__device__ __forceinline__ uint64_t same_digit_mask(bool valid, uint32_t d)
 {
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const unsigned int gen = __hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence();
-        const unsigned int arrived =
-            __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (arrived == nblocks - 1) {
-            __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_fetch_add(&bar[1], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            // relaxed polling: an acquire load per poll would invalidate this
-            // XCD's L2 every time; one acquire fence after the wait suffices
-            while (__hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen)
-                __builtin_amdgcn_s_sleep(2);
-        }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    uint64_t same = __ballot(valid);
+#pragma unroll
+    for (int bit = 0; bit < kGBits; ++bit) {
+        const uint64_t bb = __ballot((d >> bit) & 1u);
+        same &= ((d >> bit) & 1u) ? bb : ~bb;
     }
-    __syncthreads();
+    return same;
 }
 
-struct CoopArgs {
-    HistArgs h;
-    uint64_t *kbuf[2];
-    uint32_t *vbuf[2];
-    uint32_t *block_counts;          // [kGDigits][gridDim.x]
-    unsigned long long *kmax;        // zero between launches
-    unsigned int *barrier;           // [2]
+struct OsDigitsArgs {
+    const uint64_t *keys;
+    const int32_t *count;
+    unsigned int *digit_counts;   // [kMaxPasses][kGDigits]
+    int passes;
 };
 
-// All workgroups: m > kLdsCap pairs.  Workgroup b owns positions
-// [b * chunk, (b + 1) * chunk) in every pass; rows of kSortThreads positions
-// are ranked stably with wave ballots (8-bit multi-split).
-__global__ __launch_bounds__(kSortThreads) void hist_coop_kernel(CoopArgs a)
+__global__ __launch_bounds__(kOsThreads) void os_digits_kernel(OsDigitsArgs a)
 {
-    extern __shared__ unsigned char smem[];
-    const int m = *a.h.count;
-    if (m <= kLdsCap) {
-        if (m > 0 && blockIdx.x == 0) lds_sort_reduce_bin(a.h, m, smem);
-        return;   // uniform over the grid: no barrier is reached
+    __shared__ unsigned int s_c[kMaxPasses * kGDigits];
+    const int m = *a.count;
+    const int64_t base = (int64_t)blockIdx.x * kOsTile;
+    if (m <= kLdsCap || base >= m) return;
+    for (int i = threadIdx.x; i < a.passes * kGDigits; i += kOsThreads) s_c[i] = 0;
+    __syncthreads();
+    for (int r = 0; r < kOsRows; ++r) {
+        const int64_t i = base + (int64_t)r * kOsThreads + threadIdx.x;
+        if (i < m) {
+            const uint64_t k = a.keys[i];
+            for (int p = 0; p < a.passes; ++p) atomicAdd(&s_c[p * kGDigits + ((k >> (p * kGBits)) & 255)], 1u);
+        }
     }
-    const unsigned int G = gridDim.x;
-    const int b = blockIdx.x;
+    __syncthreads();
+    for (int i = threadIdx.x; i < a.passes * kGDigits; i += kOsThreads)
+        if (s_c[i]) atomicAdd(&a.digit_counts[i], s_c[i]);
+}
+
+struct OsPassArgs {
+    const uint64_t *src_k;        // null on pass 0: the pairs' keys, identity payload
+    const uint32_t *src_v;
+    const uint64_t *pair_keys;
+    uint64_t *dst_k;
+    uint32_t *dst_v;
+    const int32_t *count;
+    const unsigned int *digit_counts;   // this pass's row
+    unsigned long long *status;         // [tiles][kGDigits]
+    unsigned long long *ticket;
+    unsigned long long ticket_base;
+    uint32_t epoch;
+    int shift;
+};
+
+__global__ __launch_bounds__(kOsThreads) void os_pass_kernel(OsPassArgs a)
+{
+    __shared__ uint16_t s_cnt[kOsRows][kOsWaves][kGDigits];
+    __shared__ uint32_t s_off[kGDigits];
+    __shared__ uint32_t s_wsum[kOsWaves];
+    __shared__ int s_tile;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
-    uint32_t *s_hist = reinterpret_cast<uint32_t *>(smem);             // [kGDigits]
-    uint32_t *s_run = s_hist + kGDigits;                                 // [kGDigits]
-    uint16_t *s_wcnt = reinterpret_cast<uint16_t *>(s_run + kGDigits);  // [kSortWaves][kGDigits]
-    uint32_t *s_wave = reinterpret_cast<uint32_t *>(s_wcnt + kSortWaves * kGDigits);
-
-    int64_t chunk = ((int64_t)m + G - 1) / G;
-    chunk = (chunk + kSortThreads - 1) / kSortThreads * kSortThreads;
-    const int64_t lo = min((int64_t)m, (int64_t)b * chunk);
-    const int64_t hi = min((int64_t)m, lo + chunk);
-
-    // largest key -> number of 8-bit passes
+    // every workgroup takes its ticket (the host advances the base by the grid size)
+    if (tid == 0) s_tile = (int)(atomicAdd(a.ticket, 1ull) - a.ticket_base);
+    for (int i = tid; i < kOsRows * kOsWaves * kGDigits; i += kOsThreads) (&s_cnt[0][0][0])[i] = 0;
+    __syncthreads();
+    const int m = *a.count;
+    const int tile = s_tile;
+    const int64_t base = (int64_t)tile * kOsTile;
+    // small inputs were sorted in LDS; tiles past the data publish nothing
+    // (tickets are handed out in order, so no later tile holds data)
+    if (m <= kLdsCap || base >= m) return;
+    // exclusive scan of the pass's digit counts: start of each digit's output
+    uint32_t dbase;
     {
-        unsigned long long mx = 0;
-        for (int64_t i = lo + tid; i < hi; i += kSortThreads) {
-            const uint64_t kk = a.h.keys[i];
-            mx = kk > mx ? kk : mx;
-        }
-        for (int d = 32; d > 0; d >>= 1) {
-            const unsigned long long o = __shfl_xor(mx, d, 64);
-            mx = o > mx ? o : mx;
-        }
-        if (lane == 0 && mx) atomicMax(a.kmax, mx);
-    }
-    grid_barrier(a.barrier, G);
-    const unsigned long long kmax = __hip_atomic_load(a.kmax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const int bits = kmax ? 64 - __clzll((long long)kmax) : 0;
-    const int passes = (bits + kGBits - 1) / kGBits;
-
-    for (int i = tid; i < kSortWaves * kGDigits; i += kSortThreads) s_wcnt[i] = 0;
-    const uint64_t *src_k = nullptr;
-    const uint32_t *src_v = nullptr;
-    for (int p = 0; p < passes; ++p) {
-        const int shift = p * kGBits;
-        uint64_t *dst_k = a.kbuf[p & 1];
-        uint32_t *dst_v = a.vbuf[p & 1];
-        // phase A: digit counts of this workgroup's chunk
-        for (int d = tid; d < kGDigits; d += kSortThreads) s_hist[d] = 0;
-        __syncthreads();
-        for (int64_t i = lo + tid; i < hi; i += kSortThreads) {
-            const uint64_t kk = src_k ? src_k[i] : a.h.keys[i];
-            atomicAdd(&s_hist[(kk >> shift) & (kGDigits - 1)], 1u);
-        }
-        __syncthreads();
-        for (int d = tid; d < kGDigits; d += kSortThreads) a.block_counts[(size_t)d * G + b] = s_hist[d];
-        grid_barrier(a.barrier, G);
-        // phase B: workgroup 0 turns the digit-major table into exclusive offsets
-        if (b == 0) {
-            const int total = kGDigits * (int)G;
-            const int per = (total + kSortThreads - 1) / kSortThreads;
-            const int e0 = min(tid * per, total);
-            const int e1 = min(e0 + per, total);
-            uint32_t run = 0;
-            for (int e = e0; e < e1; ++e) run += a.block_counts[e];
-            uint32_t base = block_exclusive_scan_1024(run, s_wave);
-            for (int e = e0; e < e1; ++e) {
-                const uint32_t c = a.block_counts[e];
-                a.block_counts[e] = base;
-                base += c;
-            }
-        }
-        grid_barrier(a.barrier, G);
-        // phase C: stable scatter, one row of kSortThreads positions at a time
-        for (int d = tid; d < kGDigits; d += kSortThreads) s_run[d] = a.block_counts[(size_t)d * G + b];
-        __syncthreads();
-        const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-        for (int64_t r0 = lo; r0 < hi; r0 += kSortThreads) {
-            const int64_t i = r0 + tid;
-            const bool valid = i < hi;
-            uint64_t kk = 0;
-            uint32_t vv = 0;
-            if (valid) {
-                if (src_k) {
-                    kk = src_k[i];
-                    vv = src_v[i];
-                } else {
-                    kk = a.h.keys[i];
-                    vv = (uint32_t)i;
-                }
-            }
-            const uint32_t d = (uint32_t)((kk >> shift) & (kGDigits - 1));
-            uint64_t same = __ballot(valid);
+        const uint32_t c = a.digit_counts[tid];
+        uint32_t incl = c;
 #pragma unroll
-            for (int bit = 0; bit < kGBits; ++bit) {
-                const uint64_t bb = __ballot((d >> bit) & 1u);
-                same &= ((d >> bit) & 1u) ? bb : ~bb;
-            }
-            const uint32_t rank = (uint32_t)__popcll(same & lt_mask);
-            if (valid && rank == 0) s_wcnt[wave * kGDigits + d] = (uint16_t)__popcll(same);
-            __syncthreads();
-            if (valid) {
-                uint32_t before = 0;
-                for (int w = 0; w < wave; ++w) before += s_wcnt[w * kGDigits + d];
-                const uint32_t pos = s_run[d] + before + rank;
-                dst_k[pos] = kk;
-                dst_v[pos] = vv;
-            }
-            __syncthreads();
-            for (int dd = tid; dd < kGDigits; dd += kSortThreads) {
-                uint32_t add = 0;
-                for (int w = 0; w < kSortWaves; ++w) {
-                    add += s_wcnt[w * kGDigits + dd];
-                    s_wcnt[w * kGDigits + dd] = 0;
-                }
-                s_run[dd] += add;
-            }
-            __syncthreads();
+        for (int dd = 1; dd < 64; dd <<= 1) {
+            const uint32_t t = __shfl_up(incl, dd, 64);
+            if (lane >= dd) incl += t;
         }
-        grid_barrier(a.barrier, G);
-        src_k = dst_k;
-        src_v = dst_v;
+        if (lane == 63) s_wsum[wave] = incl;
+        __syncthreads();
+        uint32_t wb = 0;
+        for (int w = 0; w < wave; ++w) wb += s_wsum[w];
+        dbase = wb + incl - c;
     }
-    // reduce-by-key left to right and bin; runs start inside this chunk
-    for (int64_t i = lo + tid; i < hi; i += kSortThreads) {
-        const uint64_t kk = src_k ? src_k[i] : a.h.keys[i];
-        if (i > 0 && (src_k ? src_k[i - 1] : a.h.keys[i - 1]) == kk) continue;
-        double s = a.h.y[src_v ? src_v[i] : (uint32_t)i];
+    uint64_t kk[kOsRows];
+    uint32_t vv[kOsRows];
+#pragma unroll
+    for (int r = 0; r < kOsRows; ++r) {
+        const int64_t i = base + (int64_t)r * kOsThreads + tid;
+        kk[r] = 0;
+        vv[r] = 0;
+        if (i < m) {
+            kk[r] = a.src_k ? a.src_k[i] : a.pair_keys[i];
+            vv[r] = a.src_k ? a.src_v[i] : (uint32_t)i;
+        }
+    }
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    uint32_t rank[kOsRows];
+#pragma unroll
+    for (int r = 0; r < kOsRows; ++r) {
+        const int64_t i = base + (int64_t)r * kOsThreads + tid;
+        const bool valid = i < m;
+        const uint32_t d = (uint32_t)((kk[r] >> a.shift) & (kGDigits - 1));
+        const uint64_t same = same_digit_mask(valid, d);
+        rank[r] = (uint32_t)__popcll(same & lt_mask);
+        if (valid && rank[r] == 0) s_cnt[r][wave][d] = (uint16_t)__popcll(same);
+    }
+    __syncthreads();
+    // thread d: exclusive prefix of digit d over (row, wave) in tile order,
+    // then the look-back for digit d
+    {
+        const int d = tid;
+        uint32_t run = 0;
+        for (int r = 0; r < kOsRows; ++r)
+            for (int w = 0; w < kOsWaves; ++w) {
+                const uint32_t c = s_cnt[r][w][d];
+                s_cnt[r][w][d] = (uint16_t)run;
+                run += c;
+            }
+        const uint32_t ep = a.epoch & 0x3FFFFFFFu;
+        unsigned long long *st = a.status + (size_t)tile * kGDigits + d;
+        auto pack = [&](unsigned long long flag, uint32_t v) {
+            return (flag << 62) | ((unsigned long long)ep << 32) | v;
+        };
+        uint32_t excl = 0;
+        if (tile == 0) {
+            __hip_atomic_store(st, pack(2, run), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            __hip_atomic_store(st, pack(1, run), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            int j = tile - 1;
+            while (true) {
+                const unsigned long long s =
+                    __hip_atomic_load(a.status + (size_t)j * kGDigits + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const uint32_t sep = (uint32_t)(s >> 32) & 0x3FFFFFFFu;
+                const unsigned long long fl = s >> 62;
+                if (sep != ep || fl == 0) {
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                excl += (uint32_t)s;
+                if (fl == 2) break;
+                --j;
+            }
+            __hip_atomic_store(st, pack(2, excl + run), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        s_off[d] = dbase + excl;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kOsRows; ++r) {
+        const int64_t i = base + (int64_t)r * kOsThreads + tid;
+        if (i < m) {
+            const uint32_t d = (uint32_t)((kk[r] >> a.shift) & (kGDigits - 1));
+            const uint32_t pos = s_off[d] + s_cnt[r][wave][d] + rank[r];
+            a.dst_k[pos] = kk[r];
+            a.dst_v[pos] = vv[r];
+        }
+    }
+}
+
+struct OsReduceArgs {
+    HistArgs h;
+    const uint64_t *keys;     // sorted keys (null: no pass ran; use the pairs' keys)
+    const uint32_t *vals;
+    unsigned int *digit_counts;
+    int passes;
+};
+
+__global__ __launch_bounds__(256) void os_reduce_kernel(OsReduceArgs a)
+{
+    const int m = *a.h.count;
+    if (blockIdx.x == 0)
+        for (int i = threadIdx.x; i < kMaxPasses * kGDigits; i += blockDim.x) a.digit_counts[i] = 0;
+    if (m <= kLdsCap) return;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride) {
+        const uint64_t kk = a.keys ? a.keys[i] : a.h.keys[i];
+        if (i > 0 && (a.keys ? a.keys[i - 1] : a.h.keys[i - 1]) == kk) continue;
+        double s = a.h.y[a.keys ? a.vals[i] : (uint32_t)i];
         for (int64_t j = i + 1; j < m; ++j) {
-            const uint64_t kj = src_k ? src_k[j] : a.h.keys[j];
+            const uint64_t kj = a.keys ? a.keys[j] : a.h.keys[j];
             if (kj != kk) break;
-            s = s + a.h.y[src_v ? src_v[j] : (uint32_t)j];
+            s = s + a.h.y[a.keys ? a.vals[j] : (uint32_t)j];
         }
         bin_sum(a.h, s);
     }
-    if (b == 0 && tid == 0) *a.kmax = 0;   // read by every workgroup before the first phase-A barrier
 }
-
-constexpr size_t kCoopScratchBytes = (size_t)kGDigits * 2 * sizeof(uint32_t) +
-                                     (size_t)kSortWaves * kGDigits * sizeof(uint16_t) +
-                                     (size_t)kSortWaves * sizeof(uint32_t);
-static_assert(kCoopScratchBytes <= kLdsBytes, "global-phase LDS must fit the LDS-path allocation");
 
 __global__ void zero_u64_kernel(unsigned long long *p, int64_t n)
 {
@@ -589,39 +616,35 @@ int ensure_sort_attributes()
     if (done) return ST_OK;
     ST_HIP_TRY(hipFuncSetAttribute((const void *)hist_lds_kernel,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBytes));
-    ST_HIP_TRY(hipFuncSetAttribute((const void *)hist_coop_kernel,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBytes));
     done = true;
     return ST_OK;
 }
 
-// Buffers of the cooperative sort: ping-pong keys/payloads sized for `cap`
-// pairs, the digit table of the grid, the max-key cell and the barrier.
+// Onesweep buffers for `cap` pairs: ping-pong keys/payloads, look-back status
+// words per (tile, digit), the digit counts and the tile ticket.
 int ensure_sort_buffers(st_hist *h, int64_t cap)
 {
-    int rc = ensure_sort_attributes();
-    if (rc) return rc;
-    if (!h->coop_grid) {
-        int per_cu = 0;
-        ST_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)hist_coop_kernel,
-                                                                kSortThreads, kLdsBytes));
-        if (per_cu < 1) return set_error(ST_EHIP, "hist_coop_kernel cannot be resident");
-        h->coop_grid = per_cu * device_cu_count();
-        ST_HIP_TRY(hipMalloc((void **)&h->block_counts, (size_t)kGDigits * h->coop_grid * sizeof(uint32_t)));
-        ST_HIP_TRY(hipMalloc((void **)&h->kmax, sizeof(unsigned long long)));
-        ST_HIP_TRY(hipMalloc((void **)&h->barrier, 2 * sizeof(unsigned int)));
-        ST_HIP_TRY(hipMemset(h->kmax, 0, sizeof(unsigned long long)));
-        ST_HIP_TRY(hipMemset(h->barrier, 0, 2 * sizeof(unsigned int)));
+    if (!h->digit_counts) {
+        ST_HIP_TRY(hipMalloc((void **)&h->digit_counts, (size_t)kMaxPasses * kGDigits * sizeof(unsigned int)));
+        ST_HIP_TRY(hipMemset(h->digit_counts, 0, (size_t)kMaxPasses * kGDigits * sizeof(unsigned int)));
+        ST_HIP_TRY(hipMalloc((void **)&h->os_ticket, sizeof(unsigned long long)));
+        ST_HIP_TRY(hipMemset(h->os_ticket, 0, sizeof(unsigned long long)));
+        h->os_ticket_base = 0;
     }
     if (cap > h->sort_capacity) {
         (void)hipFree(h->keys_a); (void)hipFree(h->keys_b); (void)hipFree(h->vals_a); (void)hipFree(h->vals_b);
+        (void)hipFree(h->os_status);
         h->keys_a = h->keys_b = nullptr;
         h->vals_a = h->vals_b = nullptr;
+        h->os_status = nullptr;
         h->sort_capacity = 0;
+        const int64_t tiles = (cap + kOsTile - 1) / kOsTile;
         ST_HIP_TRY(hipMalloc((void **)&h->keys_a, (size_t)cap * sizeof(uint64_t)));
         ST_HIP_TRY(hipMalloc((void **)&h->keys_b, (size_t)cap * sizeof(uint64_t)));
         ST_HIP_TRY(hipMalloc((void **)&h->vals_a, (size_t)cap * sizeof(uint32_t)));
         ST_HIP_TRY(hipMalloc((void **)&h->vals_b, (size_t)cap * sizeof(uint32_t)));
+        ST_HIP_TRY(hipMalloc((void **)&h->os_status, (size_t)tiles * kGDigits * sizeof(unsigned long long)));
+        ST_HIP_TRY(hipMemset(h->os_status, 0, (size_t)tiles * kGDigits * sizeof(unsigned long long)));
         h->sort_capacity = cap;
     }
     return ST_OK;
@@ -641,25 +664,46 @@ int hist_add_pairs(st_hist *h, const st_pairs *pairs, hipStream_t stream)
     ha.edges = h->edges_dev;
     ha.guess = h->guess;
     ha.bins = (unsigned long long *)h->bins_dev;
-    if (pairs->capacity <= kLdsCap) {
-        hist_lds_kernel<<<1, kSortThreads, kLdsBytes, stream>>>(ha);
-        ST_HIP_TRY(hipGetLastError());
-        return ST_OK;
-    }
+    // small inputs (decided on the device) take the one-workgroup LDS sort
+    hist_lds_kernel<<<1, kSortThreads, kLdsBytes, stream>>>(ha);
+    ST_HIP_TRY(hipGetLastError());
+    if (pairs->capacity <= kLdsCap) return ST_OK;
     rc = ensure_sort_buffers(h, pairs->capacity);
     if (rc) return rc;
-    CoopArgs ca;
-    ca.h = ha;
-    ca.kbuf[0] = h->keys_a;
-    ca.kbuf[1] = h->keys_b;
-    ca.vbuf[0] = h->vals_a;
-    ca.vbuf[1] = h->vals_b;
-    ca.block_counts = h->block_counts;
-    ca.kmax = h->kmax;
-    ca.barrier = h->barrier;
-    void *args[] = {&ca};
-    ST_HIP_TRY(hipLaunchCooperativeKernel((const void *)hist_coop_kernel, dim3(h->coop_grid),
-                                          dim3(kSortThreads), args, kLdsBytes, stream));
+    // passes for the largest key the producer can write (all 64 bits if unknown)
+    const uint64_t kb = pairs->key_bound ? pairs->key_bound : ~0ull;
+    const int bits = 64 - __builtin_clzll(kb | 1ull);
+    const int passes = (bits + kGBits - 1) / kGBits;
+    const int tiles = (int)((pairs->capacity + kOsTile - 1) / kOsTile);
+    OsDigitsArgs da{pairs->volume, pairs->num_elements, h->digit_counts, passes};
+    os_digits_kernel<<<tiles, kOsThreads, 0, stream>>>(da);
+    const uint64_t *src_k = nullptr;
+    const uint32_t *src_v = nullptr;
+    uint64_t *bk[2] = {h->keys_a, h->keys_b};
+    uint32_t *bv[2] = {h->vals_a, h->vals_b};
+    for (int p = 0; p < passes; ++p) {
+        OsPassArgs pa;
+        pa.src_k = src_k;
+        pa.src_v = src_v;
+        pa.pair_keys = pairs->volume;
+        pa.dst_k = bk[p & 1];
+        pa.dst_v = bv[p & 1];
+        pa.count = pairs->num_elements;
+        pa.digit_counts = h->digit_counts + p * kGDigits;
+        pa.status = h->os_status;
+        pa.ticket = h->os_ticket;
+        pa.ticket_base = h->os_ticket_base;
+        pa.epoch = ++h->os_epoch;
+        if ((pa.epoch & 0x3FFFFFFFu) == 0) pa.epoch = ++h->os_epoch;
+        pa.shift = p * kGBits;
+        os_pass_kernel<<<tiles, kOsThreads, 0, stream>>>(pa);
+        h->os_ticket_base += (unsigned long long)tiles;
+        src_k = bk[p & 1];
+        src_v = bv[p & 1];
+    }
+    OsReduceArgs ra{ha, src_k, src_v, h->digit_counts, passes};
+    os_reduce_kernel<<<grid_for(pairs->capacity, 256, 2048), 256, 0, stream>>>(ra);
+    ST_HIP_TRY(hipGetLastError());
     return ST_OK;
 }
 
@@ -747,6 +791,7 @@ int st_pairs_alloc(st_pairs *p, int64_t capacity)
     const int32_t c = (int32_t)capacity;
     ST_HIP_TRY(hipMemcpy(p->num_elements, &c, sizeof c, hipMemcpyHostToDevice));
     p->capacity = capacity;
+    p->key_bound = 0;
     return ST_OK;
 }
 
@@ -878,6 +923,12 @@ int st_vcs_process_oversample(st_vcs *m, const st_track *track, st_pairs *out)
     a.num_tiles = tiles;
     score_kernel<<<tiles, kScoreBlock, 0, m->stream>>>(a);
     ST_HIP_TRY(hipGetLastError());
+    // largest key this method can write: every index is at most N
+    {
+        const unsigned long long N = (unsigned long long)m->g.n_linear;
+        const long double kb = (long double)N + (long double)N * N + (long double)N * N * N;
+        out->key_bound = kb >= 1.8e19L ? 0ull : (N + N * N + N * N * N);
+    }
     m->ticket_base += (uint64_t)tiles;
     m->oversample++;
     return ST_OK;
@@ -1049,9 +1100,9 @@ void st_hist_destroy(st_hist *h)
     (void)hipFree(h->keys_b);
     (void)hipFree(h->vals_a);
     (void)hipFree(h->vals_b);
-    (void)hipFree(h->block_counts);
-    (void)hipFree(h->kmax);
-    (void)hipFree(h->barrier);
+    (void)hipFree(h->os_status);
+    (void)hipFree(h->digit_counts);
+    (void)hipFree(h->os_ticket);
     delete h;
 }
 

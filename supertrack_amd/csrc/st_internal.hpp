@@ -58,14 +58,15 @@ struct st_hist {
     hipStream_t stream = nullptr;
     int64_t capacity = 0;            // pairs capacity of the current track
     bool in_track = false;
-    // cooperative sort buffers (pairs capacities above the LDS path's limit)
+    // onesweep sort buffers (pairs capacities above the LDS path's limit)
     int64_t sort_capacity = 0;
     uint64_t *keys_a = nullptr, *keys_b = nullptr;
     uint32_t *vals_a = nullptr, *vals_b = nullptr;
-    uint32_t *block_counts = nullptr;     // [256 digits][coop_grid]
-    unsigned long long *kmax = nullptr;
-    unsigned int *barrier = nullptr;
-    int coop_grid = 0;
+    unsigned long long *os_status = nullptr;   // look-back words [tiles][256 digits]
+    unsigned int *digit_counts = nullptr;      // [8 passes][256 digits]
+    unsigned long long *os_ticket = nullptr;
+    unsigned long long os_ticket_base = 0;
+    uint32_t os_epoch = 0;
     std::vector<uint64_t> host_counts;
 };
 

@@ -174,7 +174,7 @@ def test_batch_is_deterministic_and_tuning_invariant():
     spec = dict(w["synth"], n_max=100000)
     x, y, z, e, off = st.synth_library(spec, 24, 50)
     results = []
-    for slots, bpc, graph in [(2048, 4, True), (1024, 2, False), (4096, 1, True), (2048, 4, False)]:
+    for slots, bpc, graph in [(2048, 4, True), (2048, 2, False), (4096, 1, True), (4096, 3, False)]:
         counts, _, _, _ = gpu_run(x, y, z, e, off, [(2.5e3, 10.0)], None, 8, 3, True, True, st.ST_F32,
                                   [("log", 1000, 1e-2, 1e3)], table_slots=slots, blocks_per_cu=bpc, use_graph=graph)
         results.append(counts[0])

@@ -138,7 +138,9 @@ def test_histogram_tests_known_answers(capacity):
 @pytest.mark.parametrize("m,keyspace,capacity", [(1, 10, None), (5000, 1 << 20, None), (6144, 7, None),
                                                  (6144, 1 << 40, None), (6145, 1 << 27, None),
                                                  (250000, 1 << 27, None), (300000, 1 << 60, None),
-                                                 (4000, 1 << 27, 100000), (0, 10, 100000)])
+                                                 (4000, 1 << 27, 100000), (0, 10, 100000),
+                                                 (100000, 1 << 35, 200000), (131072, 1 << 20, None),
+                                                 (20000, 3, None)])
 def test_histogram_random_pairs_match_oracle(m, keyspace, capacity):
     rng = np.random.default_rng(m + 3)
     keys = rng.integers(0, keyspace, m, dtype=np.uint64)
