@@ -181,7 +181,7 @@ typedef struct {
     int32_t block_threads;        /* 0 = default / autotuned                   */
     int32_t table_slots;          /* LDS hash slots per workgroup: 2048 or 4096 */
     int32_t blocks_per_cu;        /* persistent workgroups per CU              */
-    int32_t pad;
+    int32_t points_per_thread;    /* 1 or 2 points per thread per iteration    */
 } st_batch_opts;
 
 typedef struct st_batch st_batch;
@@ -196,8 +196,9 @@ int st_batch_sync(st_batch *b);
 /* Device time of the last st_batch_run in ms: ms2[0] the fused kernel
  * (with its reset), ms2[1] the exact per-segment path. */
 int st_batch_times(st_batch *b, float *ms2);
-/* Launch parameters in use: block_threads, table_slots, blocks_per_cu. */
-int st_batch_params(const st_batch *b, int32_t *params3);
+/* Launch parameters in use: block_threads, table_slots, blocks_per_cu,
+ * points_per_thread. */
+int st_batch_params(const st_batch *b, int32_t *params4);
 /* Test hook: record (segment, key, sum) of every scored sphere of the next
  * st_batch_run into host-visible arrays of the given capacity (segment =
  * track * n_oversamples + oversample).  capacity 0 disables recording. */

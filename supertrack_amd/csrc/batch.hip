@@ -43,7 +43,7 @@ namespace st {
 namespace {
 
 constexpr int kBT = 256;                    // threads per workgroup
-constexpr int kBlk = 256;                   // points per library block (== chunk)
+constexpr int kBlk = 256;                   // points per library block
 constexpr unsigned long long kEmpty = ~0ull;
 constexpr int kMaxGeoms = 16;
 constexpr int kMaxHists = 2;
@@ -219,13 +219,14 @@ __device__ int wave_lower_bound(int n, Pred pred)
 // ---------------------------------------------------------------------------
 // the persistent fused kernel
 // ---------------------------------------------------------------------------
-template <int MODE, int SLOTS>
+// kPPT: library blocks (points per thread) loaded together per iteration.
+template <int MODE, int SLOTS, int kPPT>
 __global__ __launch_bounds__(kBT) void batch_kernel(BatchArgs a)
 {
     static_assert(SLOTS == 2048 || SLOTS == 4096, "table sizes");
-    // occupancy never exceeds max(half table, survivors) + one chunk
-    static_assert((SLOTS / 2 > kMaxWindow ? SLOTS / 2 : kMaxWindow) + kBlk <= SLOTS * 9 / 10,
-                  "the table must hold the survivors of a flush plus a chunk");
+    // occupancy never exceeds max(half table, survivors) + one iteration's points
+    static_assert((SLOTS / 2 > kMaxWindow ? SLOTS / 2 : kMaxWindow) + kPPT * kBlk <= SLOTS * 9 / 10,
+                  "the table must hold the survivors of a flush plus an iteration");
     constexpr int kLog2Slots = (SLOTS == 2048) ? 11 : 12;
     constexpr int kPer = SLOTS / kBT;            // table slots scanned per thread
     constexpr int kWords = Acc<MODE>::words;
@@ -362,75 +363,107 @@ __global__ __launch_bounds__(kBT) void batch_kernel(BatchArgs a)
         const int b_lo = s_range[0], b_hi = s_range[1];
         if (b_lo >= b_hi) continue;
         const int64_t p_end = td.offset + min((int64_t)b_hi * kBlk, (int64_t)td.n);
-        for (int b = b_lo; b < b_hi; ++b) {
+        for (int b = b_lo; b < b_hi; b += kPPT) {
             const int64_t c = td.offset + (int64_t)b * kBlk;
-            const int64_t i = c + tid;
-            bool in = false;
-            unsigned long long key = 0;
-            double yv = 0.0;
-            if (i < p_end) {
-                if (MODE == ST_F64) {
-                    in = exact_member(a.x64[i], a.y64[i], a.z64[i] + g.H, g, sh.x, sh.y, sh.z, key);
-                    if (in) yv = a.e64[i] / g.denom;
-                } else {
-                    const int64_t blk = td.block0 + b;
-                    const double ox = a.origin[3 * blk + 0];
-                    const double oy = a.origin[3 * blk + 1];
-                    const double oz = a.origin[3 * blk + 2];
-                    const double dl = a.eps[blk] + kTwoM44 * (fabs(ox) + fabs(oy) + fabs(oz) + fabs(sh.x) +
-                                                               fabs(sh.y) + fabs(sh.z) + 4.0 * g.H +
-                                                               fabs(g.gso) + g.D + 16.0);
-                    const double xs = (ox + sh.x) + (double)a.dx32[i];
-                    const double ys = (oy + sh.y) + (double)a.dy32[i];
-                    const double zs = ((oz + g.H) + sh.z) + (double)a.dz32[i];
-                    bool amb = fabs(fabs(xs) - g.H) <= dl || fabs(fabs(ys) - g.H) <= dl ||
-                               fabs(fabs(zs) - g.H) <= dl;
-                    if (!amb && fabs(xs) < g.H && fabs(ys) < g.H && fabs(zs) < g.H) {
-                        const double ax = xs - g.gso, ay = ys - g.gso, az = zs - g.gso;
-                        const double qx = ax * g.invD, qy = ay * g.invD, qz = az * g.invD;
-                        const double kx = rint(qx), ky = rint(qy), kz = rint(qz);
-                        const double fm = 0.5 - (dl * g.invD + 1e-12);
-                        amb = fabs(qx - kx) >= fm || fabs(qy - ky) >= fm || fabs(qz - kz) >= fm;
-                        if (!amb) {
-                            const double dx = kx * g.D - ax, dy = ky * g.D - ay, dz = kz * g.D - az;
-                            const double dist = dx * dx + dy * dy + dz * dz;
-                            const double mrg = 3.0 * dl * (g.D + 2.0 * dl) + 1e-14 * g.r2;
-                            amb = fabs(dist - g.r2) <= mrg;
-                            if (!amb && dist < g.r2) {
-                                in = true;
-                                key = (unsigned long long)(long long)kx +
-                                      (unsigned long long)(long long)ky * g.N +
-                                      (unsigned long long)(long long)kz * g.N * g.N;
-                            }
-                        }
+            // loads of all kPPT points of this thread first, then the arithmetic
+            bool valid[kPPT];
+            double px[kPPT], py[kPPT], pz[kPPT], pe[kPPT];
+            float fx[kPPT], fy[kPPT], fz[kPPT], fe[kPPT];
+            double ox[kPPT], oy[kPPT], oz[kPPT], ep[kPPT];
+#pragma unroll
+            for (int j = 0; j < kPPT; ++j) {
+                const int64_t i = c + (int64_t)j * kBlk + tid;
+                valid[j] = (b + j < b_hi) && i < p_end;
+                if constexpr (MODE == ST_F64) {
+                    if (valid[j]) {
+                        px[j] = a.x64[i];
+                        py[j] = a.y64[i];
+                        pz[j] = a.z64[i];
+                        pe[j] = a.e64[i];
                     }
-                    if (amb) in = exact_member(a.x64[i], a.y64[i], a.z64[i] + g.H, g, sh.x, sh.y, sh.z, key);
-                    if (in) yv = (double)a.e32[i] * g.denom_inv;
+                } else {
+                    if (valid[j]) {
+                        fx[j] = a.dx32[i];
+                        fy[j] = a.dy32[i];
+                        fz[j] = a.dz32[i];
+                        fe[j] = a.e32[i];
+                    }
+                    if (b + j < b_hi) {
+                        const int64_t blk = td.block0 + b + j;
+                        ox[j] = a.origin[3 * blk + 0];
+                        oy[j] = a.origin[3 * blk + 1];
+                        oz[j] = a.origin[3 * blk + 2];
+                        ep[j] = a.eps[blk];
+                    }
                 }
             }
-            if (in) {
-                long long v0, v1;
-                Acc<MODE>::split(yv, v0, v1);
-                unsigned int h = slot_hash(key, kLog2Slots);
-                int probe = 0;
-                for (; probe < SLOTS; ++probe) {
-                    const unsigned long long prev = atomicCAS(&tkey[h], kEmpty, key);
-                    if (prev == kEmpty) {
-                        atomicAdd(&s_occ, 1);
-                        break;
+#pragma unroll
+            for (int j = 0; j < kPPT; ++j) {
+                const int64_t i = c + (int64_t)j * kBlk + tid;
+                bool in = false;
+                unsigned long long key = 0;
+                double yv = 0.0;
+                if (valid[j]) {
+                    if constexpr (MODE == ST_F64) {
+                        in = exact_member(px[j], py[j], pz[j] + g.H, g, sh.x, sh.y, sh.z, key);
+                        if (in) yv = pe[j] / g.denom;
+                    } else {
+                        const double dl = ep[j] + kTwoM44 * (fabs(ox[j]) + fabs(oy[j]) + fabs(oz[j]) + fabs(sh.x) +
+                                                             fabs(sh.y) + fabs(sh.z) + 4.0 * g.H + fabs(g.gso) +
+                                                             g.D + 16.0);
+                        const double xs = (ox[j] + sh.x) + (double)fx[j];
+                        const double ys = (oy[j] + sh.y) + (double)fy[j];
+                        const double zs = ((oz[j] + g.H) + sh.z) + (double)fz[j];
+                        bool amb = fabs(fabs(xs) - g.H) <= dl || fabs(fabs(ys) - g.H) <= dl ||
+                                   fabs(fabs(zs) - g.H) <= dl;
+                        if (!amb && fabs(xs) < g.H && fabs(ys) < g.H && fabs(zs) < g.H) {
+                            const double ax = xs - g.gso, ay = ys - g.gso, az = zs - g.gso;
+                            const double qx = ax * g.invD, qy = ay * g.invD, qz = az * g.invD;
+                            const double kx = rint(qx), ky = rint(qy), kz = rint(qz);
+                            const double fm = 0.5 - (dl * g.invD + 1e-12);
+                            amb = fabs(qx - kx) >= fm || fabs(qy - ky) >= fm || fabs(qz - kz) >= fm;
+                            if (!amb) {
+                                const double dx = kx * g.D - ax, dy = ky * g.D - ay, dz = kz * g.D - az;
+                                const double dist = dx * dx + dy * dy + dz * dz;
+                                const double mrg = 3.0 * dl * (g.D + 2.0 * dl) + 1e-14 * g.r2;
+                                amb = fabs(dist - g.r2) <= mrg;
+                                if (!amb && dist < g.r2) {
+                                    in = true;
+                                    key = (unsigned long long)(long long)kx +
+                                          (unsigned long long)(long long)ky * g.N +
+                                          (unsigned long long)(long long)kz * g.N * g.N;
+                                }
+                            }
+                        }
+                        // any decision within the error bound: redo it exactly from FP64
+                        if (amb) in = exact_member(a.x64[i], a.y64[i], a.z64[i] + g.H, g, sh.x, sh.y, sh.z, key);
+                        if (in) yv = (double)fe[j] * g.denom_inv;
                     }
-                    if (prev == key) break;
-                    h = (h + 1) & (SLOTS - 1);
                 }
-                if (probe == SLOTS) {
-                    atomicOr(a.error, 1);
-                } else {
-                    atomicAdd(&tacc0[h], (unsigned long long)v0);
-                    if (kWords == 2) atomicAdd(&tacc1[h], (unsigned long long)v1);
+                if (in) {
+                    long long v0, v1;
+                    Acc<MODE>::split(yv, v0, v1);
+                    unsigned int h = slot_hash(key, kLog2Slots);
+                    int probe = 0;
+                    for (; probe < SLOTS; ++probe) {
+                        const unsigned long long prev = atomicCAS(&tkey[h], kEmpty, key);
+                        if (prev == kEmpty) {
+                            atomicAdd(&s_occ, 1);
+                            break;
+                        }
+                        if (prev == key) break;
+                        h = (h + 1) & (SLOTS - 1);
+                    }
+                    if (probe == SLOTS) {
+                        atomicOr(a.error, 1);
+                    } else {
+                        atomicAdd(&tacc0[h], (unsigned long long)v0);
+                        if (kWords == 2) atomicAdd(&tacc1[h], (unsigned long long)v1);
+                    }
                 }
             }
             __syncthreads();
-            const int64_t next = c + kBlk;
+            const int64_t next = c + (int64_t)kPPT * kBlk;
             if (s_occ >= SLOTS / 2 && next < p_end) {
                 // every later point lies in a layer >= the next point's exact
                 // layer index (z order), and a key's layer (key / N^2) is never
@@ -484,16 +517,20 @@ __global__ void batch_reset_kernel(unsigned long long *ticket, int *error, unsig
     }
 }
 
-template <int MODE, int SLOTS>
+template <int MODE, int SLOTS, int PPT>
 const void *kernel_ptr()
 {
-    return (const void *)batch_kernel<MODE, SLOTS>;
+    return (const void *)batch_kernel<MODE, SLOTS, PPT>;
 }
 
-const void *pick_kernel(int mode, int slots)
+const void *pick_kernel(int mode, int slots, int ppt)
 {
-    if (mode == ST_F64) return slots == 2048 ? kernel_ptr<ST_F64, 2048>() : kernel_ptr<ST_F64, 4096>();
-    return slots == 2048 ? kernel_ptr<ST_F32, 2048>() : kernel_ptr<ST_F32, 4096>();
+    if (mode == ST_F64) {
+        if (slots == 2048) return ppt == 2 ? kernel_ptr<ST_F64, 2048, 2>() : kernel_ptr<ST_F64, 2048, 1>();
+        return ppt == 2 ? kernel_ptr<ST_F64, 4096, 2>() : kernel_ptr<ST_F64, 4096, 1>();
+    }
+    if (slots == 2048) return ppt == 2 ? kernel_ptr<ST_F32, 2048, 2>() : kernel_ptr<ST_F32, 2048, 1>();
+    return ppt == 2 ? kernel_ptr<ST_F32, 4096, 2>() : kernel_ptr<ST_F32, 4096, 1>();
 }
 
 size_t lds_bytes(int mode, int slots, int hist_words)
@@ -577,7 +614,7 @@ struct st_batch {
     st_batch_opts opts{};
     std::vector<st_hist *> hists;
     hipStream_t stream = nullptr;
-    int threads = kBT, slots = 2048, blocks_per_cu = 4, grid = 0;
+    int threads = kBT, slots = 2048, blocks_per_cu = 4, grid = 0, ppt = 1;
     int hist_words = 0;
     bool lds_hist = true;     // histograms privatised in LDS (else device atomics)
     size_t lds = 0;
@@ -652,21 +689,22 @@ int launch_fast(st_batch *b, int slots, int grid)
     batch_reset_kernel<<<1, 64, 0, b->stream>>>(b->ticket, b->error, b->rec_cap > 0 ? b->rec_count : nullptr);
     if (a.n_seg == 0) return ST_OK;
     const size_t lds = lds_bytes(b->lib->precision, slots, b->lds_hist ? b->hist_words : 0);
-    const void *k = pick_kernel(b->lib->precision, slots);
+    const void *k = pick_kernel(b->lib->precision, slots, b->ppt);
     void *args[] = {&a};
     ST_HIP_TRY(hipLaunchKernel(k, dim3(grid), dim3(kBT), args, lds, b->stream));
     return ST_OK;
 }
 
-int configure(st_batch *b, int slots, int bpc)
+int configure(st_batch *b, int slots, int bpc, int ppt)
 {
+    b->ppt = ppt == 2 ? 2 : 1;
     // privatise the histograms in LDS when they fit next to the table at two
     // workgroups per CU; otherwise count straight into the device bins
     b->lds_hist = lds_bytes(b->lib->precision, slots, b->hist_words) <= 80 * 1024;
     const size_t lds = lds_bytes(b->lib->precision, slots, b->lds_hist ? b->hist_words : 0);
     if (lds > 160 * 1024)
         return set_error(ST_ERANGE, "batched kernel needs %zu B of LDS (limit 160 KiB)", lds);
-    const void *k = pick_kernel(b->lib->precision, slots);
+    const void *k = pick_kernel(b->lib->precision, slots, b->ppt);
     ST_HIP_TRY(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     int per_cu = 0;
     ST_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, kBT, lds));
@@ -979,7 +1017,9 @@ int st_batch_create(st_library *lib, const st_batch_opts *opts, st_hist *const *
     if (slots != 2048 && slots != 4096)
         return fail(set_error(ST_EINVAL, "st_batch_create: table_slots must be 2048 or 4096"));
     int bpc = opts->blocks_per_cu ? opts->blocks_per_cu : 4;
-    rc = configure(b, slots, bpc);
+    const int ppt = opts->points_per_thread ? opts->points_per_thread : 1;
+    if (ppt != 1 && ppt != 2) return fail(set_error(ST_EINVAL, "st_batch_create: points_per_thread must be 1 or 2"));
+    rc = configure(b, slots, bpc, ppt);
     if (rc) return fail(rc);
     if (opts->autotune && !lib->fast_order.empty() && opts->n_oversamples > 0) {
         // time each candidate on one full pass; keep the fastest
@@ -989,16 +1029,17 @@ int st_batch_create(st_library *lib, const st_batch_opts *opts, st_hist *const *
         ST_HIP_TRY(hipEventCreate(&e0));
         ST_HIP_TRY(hipEventCreate(&e1));
         float best = 1e30f;
-        int best_s = slots, best_b = b->blocks_per_cu;
+        int best_s = slots, best_b = b->blocks_per_cu, best_p = b->ppt;
         std::vector<std::vector<uint64_t>> saved;
         for (st_hist *h : b->hists) {
             saved.emplace_back(h->nbins);
             ST_HIP_TRY(hipMemcpy(saved.back().data(), h->bins_dev, h->nbins * sizeof(uint64_t),
                                  hipMemcpyDeviceToHost));
         }
-        for (int s : cand_slots) {
+        for (int s : cand_slots)
+        for (int pp = 1; pp <= 2; ++pp) {
             for (int bp : cand_bpc) {
-                if (configure(b, s, bp) != ST_OK) continue;
+                if (configure(b, s, bp, pp) != ST_OK) continue;
                 if (b->blocks_per_cu != bp) continue;
                 ST_HIP_TRY(hipEventRecord(e0, b->stream));
                 rc = launch_fast(b, b->slots, b->grid);
@@ -1011,6 +1052,7 @@ int st_batch_create(st_library *lib, const st_batch_opts *opts, st_hist *const *
                     best = ms;
                     best_s = s;
                     best_b = bp;
+                    best_p = pp;
                 }
             }
         }
@@ -1019,7 +1061,7 @@ int st_batch_create(st_library *lib, const st_batch_opts *opts, st_hist *const *
                                  hipMemcpyHostToDevice));
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
-        rc = configure(b, best_s, best_b);
+        rc = configure(b, best_s, best_b, best_p);
         if (rc) return fail(rc);
     }
     *out = b;
@@ -1100,6 +1142,7 @@ int st_batch_params(const st_batch *b, int32_t *p)
     p[0] = b->threads;
     p[1] = b->slots;
     p[2] = b->blocks_per_cu;
+    p[3] = b->ppt;
     return ST_OK;
 }
 

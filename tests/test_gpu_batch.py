@@ -96,7 +96,8 @@ def test_batch_cfg3_like_subset(precision, rtol):
     geoms = [po.geometry(2.5e3, 10.0)]
     oracle = oracle_segments(x, y, z, e, off, geoms, None, n_os, W.SEED, True, True)
     counts, rec, info, edges = gpu_run(x, y, z, e, off, [(2.5e3, 10.0)], None, n_os, W.SEED, True, True,
-                                       precision, [("log", 1000, 1e-2, 1e3)], record=400000)
+                                       precision, [("log", 1000, 1e-2, 1e3)], record=400000,
+                                       points_per_thread=1 + precision)
     assert info["exact_tracks"] == 0
     n, worst = compare_records(rec, oracle, rtol)
     assert n > 10000
@@ -174,9 +175,10 @@ def test_batch_is_deterministic_and_tuning_invariant():
     spec = dict(w["synth"], n_max=100000)
     x, y, z, e, off = st.synth_library(spec, 24, 50)
     results = []
-    for slots, bpc, graph in [(2048, 4, True), (2048, 2, False), (4096, 1, True), (4096, 3, False)]:
+    for slots, bpc, graph, ppt in [(2048, 4, True, 1), (2048, 2, False, 2), (4096, 1, True, 2), (4096, 3, False, 1)]:
         counts, _, _, _ = gpu_run(x, y, z, e, off, [(2.5e3, 10.0)], None, 8, 3, True, True, st.ST_F32,
-                                  [("log", 1000, 1e-2, 1e3)], table_slots=slots, blocks_per_cu=bpc, use_graph=graph)
+                                  [("log", 1000, 1e-2, 1e3)], table_slots=slots, blocks_per_cu=bpc, use_graph=graph,
+                                  points_per_thread=ppt)
         results.append(counts[0])
     for r in results[1:]:
         np.testing.assert_array_equal(r, results[0])
