@@ -29,6 +29,15 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per launch of the batched kernel, measured with rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE on the workload named by the key (workload, tracks,
+# oversamples, table slots, points per thread).  FETCH_SIZE reads 0.5 KiB per
+# KiB that a plain 1 GiB copy reads (scale 2.0); WRITE_SIZE reads 1 KiB per KiB
+# written (scale 1.0).  Other workloads or launch shapes report null.
+MEASURED_TRAFFIC = {
+    ("cfg4", 4096, 1000, 2048, 1): 2.0 * 25184634.0 * 1024 + 1.0 * 132857.8 * 1024,
+}
+
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
@@ -48,6 +57,8 @@ def parse():
     p.add_argument("--cpu-oversamples", type=int, default=0, help="CPU sample oversamples (0: workload's)")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--autotune", action="store_true")
+    p.add_argument("--copy-probe", action="store_true",
+                   help="only run a 1 GiB device-to-device copy 5 times (calibrates HBM byte counters)")
     return p.parse_args()
 
 
@@ -186,12 +197,30 @@ def cpu_baseline(w, stride, n_os):
     return deposits / total, threads, n_sample, deposits, total
 
 
+def copy_probe():
+    """1 GiB device-to-device copies: a kernel of known traffic (1 GiB read, 1 GiB written)."""
+    import supertrack_amd as st
+    n = 1 << 30
+    a, b = st.DeviceArray(n), st.DeviceArray(n)
+    st.call("st_memset", a.ptr, C.c_int(1), C.c_size_t(n))
+    for _ in range(5):
+        t0 = time.perf_counter()
+        st.call("st_memcpy_dtod", b.ptr, a.ptr, C.c_size_t(n))
+        dt = time.perf_counter() - t0
+        log(f"copy 1 GiB: {dt * 1e3:.3f} ms (host clock, synchronous)")
+    a.free()
+    b.free()
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe()
+        return
     import supertrack_amd as st
     from supertrack_amd import workloads as W
 
@@ -210,6 +239,8 @@ def main():
     log(f"{args.workload} batched: {secs * 1e3:.1f} ms/step, kernel {kms:.1f} ms, {value:.3e} deposits/s, "
         f"params {params}")
     lib.free()
+    key = (args.workload, n_tracks, n_os, params.get("table_slots"), params.get("points_per_thread"))
+    traffic = MEASURED_TRAFFIC.get(key)
 
     extra = {}
     if not args.no_compare:
@@ -266,8 +297,9 @@ def main():
                                f"geometries {w['geometries']}, histograms {w['hists']}",
                    "deposits_per_step": deposits, "parallelism": "1 GPU", "launch": params},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "basis": f"{bytes_per_dep} B per deposit x {deposits} deposits / batch kernel time"},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "basis": f"{bytes_per_dep} B per deposit x {deposits} deposits / batch kernel time "
+                              f"({kms:.2f} ms average over {args.steps} launches, HIP events on the batch stream)"},
         "cpu_baseline": cpu,
     }
     out.update(extra)

@@ -38,6 +38,7 @@
 
 #include "st_common.hpp"
 #include "st_internal.hpp"
+#include "st_device.hpp"
 
 namespace st {
 namespace {
@@ -59,6 +60,12 @@ constexpr double kTwoM64 = 1.0 / 18446744073709551616.0;
 constexpr double kTwoM32 = 1.0 / 4294967296.0;
 constexpr double kTwoM44 = 1.0 / 17592186044416.0;
 
+// Per-point lineal energy limits of the fused kernel (the fixed-point
+// conversions below are exact only under 2^51): ST_F32 y * 2^32, ST_F64
+// y * 2^24.
+constexpr double kMaxPointY32 = 262144.0;        // 2^18
+constexpr double kMaxPointY64 = 67108864.0;      // 2^26
+
 struct TrackDesc {
     int64_t offset;     // first point in the sorted arrays
     int64_t block0;     // first block in the block arrays
@@ -71,7 +78,16 @@ struct TrackDesc {
 
 struct GeomDev {
     double H, L, D, invD, r2, gso, denom, denom_inv;
+    double yscale;      // denom_inv * 2^32 (ST_F32 fixed point)
     unsigned long long N, N2;
+};
+
+// ST_F32 block record: FP64 origin of the block's FP32 offsets and the bound
+// dl on |approximate - reference| for any shifted, offset coordinate of the
+// block (FP32 rounding of the offsets plus FP64 rounding of every step, for
+// any shift of magnitude <= H').
+struct BlockF32 {
+    double ox, oy, oz, dl;
 };
 
 struct HistDev {
@@ -87,8 +103,7 @@ struct BatchArgs {
     const double *x64, *y64, *z64, *e64;
     const float *dx32, *dy32, *dz32, *e32;
     const double *zfirst;
-    const double *origin;   // 3 per block
-    const double *eps;      // per block
+    const BlockF32 *blocks;
     const GeomDev *geoms;
     // work
     const int32_t *order;
@@ -111,6 +126,25 @@ struct BatchArgs {
     unsigned long long *rec_key;
     double *rec_sum;
 };
+
+__device__ __forceinline__ int uniform_i32(int v)
+{
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
+__device__ __forceinline__ long long uniform_i64(long long v)
+{
+    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)v);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)((unsigned long long)v >> 32));
+    return (long long)(((unsigned long long)hi << 32) | lo);
+}
+
+// key = kx + ky * N + kz * N^2 (mod 2^64, as the reference's unsigned sum)
+__device__ __forceinline__ unsigned long long lattice_key(unsigned kx, unsigned ky, unsigned kz, unsigned N)
+{
+    const unsigned long long kyz = (unsigned long long)kz * N + ky;
+    return kyz * N + kx;
+}
 
 // ---------------------------------------------------------------------------
 // exact FP64 membership (the reference's operations, rounding as written)
@@ -140,8 +174,8 @@ __device__ __forceinline__ bool exact_member(double x, double y, double zp, cons
     const double dz = (kz * g.D) - az;
     const double dist = ((dx * dx) + (dy * dy)) + (dz * dz);
     if (!(dist <= g.r2)) return false;
-    key = (unsigned long long)(long long)kx + (unsigned long long)(long long)ky * g.N +
-          (unsigned long long)(long long)kz * g.N * g.N;
+    // in-voxel indices lie in [0, N] with N < 2^31
+    key = lattice_key((unsigned)(int)kx, (unsigned)(int)ky, (unsigned)(int)kz, (unsigned)g.N);
     return true;
 }
 
@@ -157,12 +191,13 @@ struct Acc;
 template <>
 struct Acc<ST_F64> {
     static constexpr int words = 2;
+    // y * 2^24 < 2^51 (kMaxPointY64)
     __device__ static void split(double y, long long &a0, long long &a1)
     {
         const double t = y * kTwo24;
         const double fh = floor(t);
-        a0 = (long long)fh;
-        a1 = (long long)rint((t - fh) * kTwo40);
+        a0 = rint_to_i64(fh);
+        a1 = rint_to_i64((t - fh) * kTwo40);
     }
     __device__ static double value(unsigned long long a0, unsigned long long a1)
     {
@@ -173,20 +208,17 @@ struct Acc<ST_F64> {
 template <>
 struct Acc<ST_F32> {
     static constexpr int words = 1;
-    __device__ static void split(double y, long long &a0, long long &a1)
-    {
-        a0 = (long long)rint(y * kTwo32);
-        a1 = 0;
-    }
     __device__ static double value(unsigned long long a0, unsigned long long)
     {
         return (double)(long long)a0 * kTwoM32;
     }
 };
 
+// 32-bit multiplicative hash of a 64-bit key
 __device__ __forceinline__ unsigned int slot_hash(unsigned long long key, int log2_slots)
 {
-    return (unsigned int)((key * 0x9E3779B97F4A7C15ull) >> (64 - log2_slots));
+    const unsigned x = (unsigned)key ^ ((unsigned)(key >> 32) * 0x85EBCA6Bu);
+    return (x * 0x9E3779B1u) >> (32 - log2_slots);
 }
 
 // ---------------------------------------------------------------------------
@@ -239,7 +271,6 @@ __global__ __launch_bounds__(kBT) void batch_kernel(BatchArgs a)
     __shared__ long long s_seg;
     __shared__ int s_occ;
     __shared__ int s_range[2];
-    __shared__ unsigned long long s_binned;
 
     const int tid = threadIdx.x;
     for (int s = tid; s < SLOTS; s += kBT) {
@@ -248,89 +279,115 @@ __global__ __launch_bounds__(kBT) void batch_kernel(BatchArgs a)
         if (kWords == 2) tacc1[s] = 0;
     }
     for (int w = tid; w < a.hist_lds_words; w += kBT) lhist[w] = 0;
-    if (tid == 0) {
-        s_occ = 0;
-        s_binned = 0;
-    }
+    if (tid == 0) s_occ = 0;
     __syncthreads();
+    // upper bound on the counts added to any LDS bin since the last spill
+    long long binned = 0;
 
-    // bins one complete sphere sum into every histogram
-    auto emit = [&](unsigned long long key, unsigned long long a0, unsigned long long a1, int64_t seg) {
-        const double v = Acc<MODE>::value(a0, a1);
+    // adds the wave's count of new table entries to the occupancy
+    auto count_fresh = [&](int n) {
+        int wsum = n;
 #pragma unroll
-        for (int h = 0; h < kMaxHists; ++h) {
-            if (h < a.n_hists) {
-                const HistDev &hd = a.hist[h];
-                const int b = bin_of(hd.edges, hd.guess.nbins, v, bin_guess(hd.guess, v));
-                if (b >= 0) {
-                    if (a.hist_lds_words) atomicAdd(&lhist[hd.lds_offset + b], 1u);
-                    else atomicAdd(&hd.bins[b], 1ull);
-                }
-            }
-        }
-        if (a.rec_cap > 0) {
-            const unsigned long long p = atomicAdd(a.rec_count, 1ull);
-            if ((int64_t)p < a.rec_cap) {
-                a.rec_seg[p] = seg;
-                a.rec_key[p] = key;
-                a.rec_sum[p] = v;
-            }
-        }
+        for (int d = 32; d >= 1; d >>= 1) wsum += __shfl_xor(wsum, d, 64);
+        if ((tid & 63) == 0 && wsum) atomicAdd(&s_occ, wsum);
     };
 
-    // flushes entries with key < bound (all entries when bound == kEmpty)
+    // Flushes entries with key < bound (all entries when bound == kEmpty):
+    // complete sums are converted to lineal energy and binned; the rest are
+    // re-inserted into the cleared table.  The bin edges of a thread's slots
+    // are loaded together before any is compared, so the global-load latency
+    // is paid once per group of slots rather than once per sphere.
     auto flush = [&](unsigned long long bound, int64_t seg) {
         __syncthreads();
         unsigned long long kk[kPer], k0[kPer], k1[kPer];
-        int nbin = 0;
+        constexpr int kGroup = 2;
 #pragma unroll
-        for (int j = 0; j < kPer; ++j) {
-            const int s = tid + j * kBT;
-            const unsigned long long k = tkey[s];
-            kk[j] = kEmpty;
-            if (k != kEmpty) {
+        for (int j0 = 0; j0 < kPer; j0 += kGroup) {
+            double vv[kGroup];
+            bool em[kGroup];
+#pragma unroll
+            for (int u = 0; u < kGroup; ++u) {
+                const int j = j0 + u;
+                const int s = tid + j * kBT;
+                const unsigned long long k = tkey[s];
                 const unsigned long long v0 = tacc0[s];
                 const unsigned long long v1 = (kWords == 2) ? tacc1[s] : 0ull;
-                if (bound == kEmpty || k < bound) {
-                    emit(k, v0, v1, seg);
-                    ++nbin;
-                } else {
-                    kk[j] = k;
-                    k0[j] = v0;
-                    k1[j] = v1;
+                const bool full = k != kEmpty;
+                em[u] = full && (bound == kEmpty || k < bound);
+                kk[j] = (full && !em[u]) ? k : kEmpty;
+                k0[j] = v0;
+                k1[j] = v1;
+                vv[u] = Acc<MODE>::value(v0, v1);
+                if (full) {
+                    tkey[s] = kEmpty;
+                    tacc0[s] = 0;
+                    if (kWords == 2) tacc1[s] = 0;
                 }
-                tkey[s] = kEmpty;
-                tacc0[s] = 0;
-                if (kWords == 2) tacc1[s] = 0;
+                if (em[u] && a.rec_cap > 0) {
+                    const unsigned long long p = atomicAdd(a.rec_count, 1ull);
+                    if ((int64_t)p < a.rec_cap) {
+                        a.rec_seg[p] = seg;
+                        a.rec_key[p] = k;
+                        a.rec_sum[p] = vv[u];
+                    }
+                }
+            }
+#pragma unroll
+            for (int h = 0; h < kMaxHists; ++h) {
+                if (h >= a.n_hists) break;
+                const HistDev &hd = a.hist[h];
+                int bb[kGroup];
+                double lo[kGroup], hi[kGroup];
+#pragma unroll
+                for (int u = 0; u < kGroup; ++u) {
+                    // closed-form guess, clamped into [0, nbins - 1]: always a valid load
+                    const double v = vv[u];
+                    float t;
+                    if (hd.guess.type == ST_HIST_LIN) t = (float)((v - hd.guess.origin) * hd.guess.inv_width);
+                    else t = (__log2f((float)v) - hd.guess.log2_origin) * hd.guess.log2_inv_width;
+                    bb[u] = (t > 0.0f) ? (int)fminf(t, (float)(hd.guess.nbins - 1)) : 0;
+                    lo[u] = hd.edges[bb[u]];
+                    hi[u] = hd.edges[bb[u] + 1];
+                }
+#pragma unroll
+                for (int u = 0; u < kGroup; ++u) {
+                    if (!em[u]) continue;
+                    const double v = vv[u];
+                    int b = bb[u];
+                    if (!(v >= lo[u] && v < hi[u])) b = bin_fast(hd.edges, hd.guess, v);
+                    if (b >= 0) {
+                        if (a.hist_lds_words) atomicAdd(&lhist[hd.lds_offset + b], 1u);
+                        else atomicAdd(&hd.bins[b], 1ull);
+                    }
+                }
             }
         }
-        if (nbin) atomicAdd(&s_binned, (unsigned long long)nbin);
         __syncthreads();
         if (tid == 0) s_occ = 0;
         __syncthreads();
+        int fresh = 0;
 #pragma unroll
         for (int j = 0; j < kPer; ++j) {
             if (kk[j] != kEmpty) {
                 unsigned int h = slot_hash(kk[j], kLog2Slots);
                 for (int probe = 0; probe < SLOTS; ++probe) {
                     const unsigned long long prev = atomicCAS(&tkey[h], kEmpty, kk[j]);
-                    if (prev == kEmpty) {
-                        atomicAdd(&s_occ, 1);
-                        break;
-                    }
+                    if (prev == kEmpty) break;
                     h = (h + 1) & (SLOTS - 1);
                 }
+                ++fresh;
                 atomicAdd(&tacc0[h], k0[j]);
                 if (kWords == 2) atomicAdd(&tacc1[h], k1[j]);
             }
         }
+        count_fresh(fresh);
         __syncthreads();
     };
 
     for (;;) {
         if (tid == 0) s_seg = (long long)atomicAdd(a.ticket, 1ull);
         __syncthreads();
-        const int64_t seg = s_seg;
+        const int64_t seg = uniform_i64(s_seg);
         __syncthreads();
         if (seg >= a.n_seg) break;
         const int32_t t = a.order[seg / a.n_os];
@@ -360,16 +417,17 @@ __global__ __launch_bounds__(kBT) void batch_kernel(BatchArgs a)
             }
         }
         __syncthreads();
-        const int b_lo = s_range[0], b_hi = s_range[1];
+        const int b_lo = uniform_i32(s_range[0]), b_hi = uniform_i32(s_range[1]);
         if (b_lo >= b_hi) continue;
         const int64_t p_end = td.offset + min((int64_t)b_hi * kBlk, (int64_t)td.n);
+        binned += (long long)(b_hi - b_lo) * kBlk;
         for (int b = b_lo; b < b_hi; b += kPPT) {
             const int64_t c = td.offset + (int64_t)b * kBlk;
             // loads of all kPPT points of this thread first, then the arithmetic
             bool valid[kPPT];
             double px[kPPT], py[kPPT], pz[kPPT], pe[kPPT];
             float fx[kPPT], fy[kPPT], fz[kPPT], fe[kPPT];
-            double ox[kPPT], oy[kPPT], oz[kPPT], ep[kPPT];
+            BlockF32 bk[kPPT];
 #pragma unroll
             for (int j = 0; j < kPPT; ++j) {
                 const int64_t i = c + (int64_t)j * kBlk + tid;
@@ -388,80 +446,75 @@ __global__ __launch_bounds__(kBT) void batch_kernel(BatchArgs a)
                         fz[j] = a.dz32[i];
                         fe[j] = a.e32[i];
                     }
-                    if (b + j < b_hi) {
-                        const int64_t blk = td.block0 + b + j;
-                        ox[j] = a.origin[3 * blk + 0];
-                        oy[j] = a.origin[3 * blk + 1];
-                        oz[j] = a.origin[3 * blk + 2];
-                        ep[j] = a.eps[blk];
-                    }
+                    if (b + j < b_hi) bk[j] = a.blocks[td.block0 + b + j];
                 }
             }
+            int fresh = 0;
 #pragma unroll
             for (int j = 0; j < kPPT; ++j) {
                 const int64_t i = c + (int64_t)j * kBlk + tid;
                 bool in = false;
                 unsigned long long key = 0;
-                double yv = 0.0;
+                long long v0 = 0, v1 = 0;
                 if (valid[j]) {
                     if constexpr (MODE == ST_F64) {
                         in = exact_member(px[j], py[j], pz[j] + g.H, g, sh.x, sh.y, sh.z, key);
-                        if (in) yv = pe[j] / g.denom;
+                        if (in) Acc<ST_F64>::split(pe[j] / g.denom, v0, v1);
                     } else {
-                        const double dl = ep[j] + kTwoM44 * (fabs(ox[j]) + fabs(oy[j]) + fabs(oz[j]) + fabs(sh.x) +
-                                                             fabs(sh.y) + fabs(sh.z) + 4.0 * g.H + fabs(g.gso) +
-                                                             g.D + 16.0);
-                        const double xs = (ox[j] + sh.x) + (double)fx[j];
-                        const double ys = (oy[j] + sh.y) + (double)fy[j];
-                        const double zs = ((oz[j] + g.H) + sh.z) + (double)fz[j];
-                        bool amb = fabs(fabs(xs) - g.H) <= dl || fabs(fabs(ys) - g.H) <= dl ||
-                                   fabs(fabs(zs) - g.H) <= dl;
-                        if (!amb && fabs(xs) < g.H && fabs(ys) < g.H && fabs(zs) < g.H) {
+                        // block-uniform terms
+                        const double dl = bk[j].dl;
+                        const double cx = bk[j].ox + sh.x;
+                        const double cy = bk[j].oy + sh.y;
+                        const double cz = (bk[j].oz + g.H) + sh.z;
+                        const double h_in = g.H - dl, h_out = g.H + dl;
+                        const double fm = 0.5 - (dl * g.invD + 1e-12);
+                        const double mrg = 3.0 * dl * (g.D + 2.0 * dl) + 1e-14 * g.r2;
+                        // per point: approximate shifted coordinates, |error| <= dl
+                        const double xs = cx + (double)fx[j];
+                        const double ys = cy + (double)fy[j];
+                        const double zs = cz + (double)fz[j];
+                        const double axs = fabs(xs), ays = fabs(ys), azs = fabs(zs);
+                        const bool box_in = axs < h_in && ays < h_in && azs < h_in;
+                        bool amb = !box_in && !(axs > h_out || ays > h_out || azs > h_out);
+                        if (box_in) {
                             const double ax = xs - g.gso, ay = ys - g.gso, az = zs - g.gso;
                             const double qx = ax * g.invD, qy = ay * g.invD, qz = az * g.invD;
-                            const double kx = rint(qx), ky = rint(qy), kz = rint(qz);
-                            const double fm = 0.5 - (dl * g.invD + 1e-12);
+                            const double tx = qx + kRoundMagic, ty = qy + kRoundMagic, tz = qz + kRoundMagic;
+                            const double kx = tx - kRoundMagic, ky = ty - kRoundMagic, kz = tz - kRoundMagic;
                             amb = fabs(qx - kx) >= fm || fabs(qy - ky) >= fm || fabs(qz - kz) >= fm;
-                            if (!amb) {
-                                const double dx = kx * g.D - ax, dy = ky * g.D - ay, dz = kz * g.D - az;
-                                const double dist = dx * dx + dy * dy + dz * dz;
-                                const double mrg = 3.0 * dl * (g.D + 2.0 * dl) + 1e-14 * g.r2;
-                                amb = fabs(dist - g.r2) <= mrg;
-                                if (!amb && dist < g.r2) {
-                                    in = true;
-                                    key = (unsigned long long)(long long)kx +
-                                          (unsigned long long)(long long)ky * g.N +
-                                          (unsigned long long)(long long)kz * g.N * g.N;
-                                }
+                            const double dx = kx * g.D - ax, dy = ky * g.D - ay, dz = kz * g.D - az;
+                            const double dist = dx * dx + dy * dy + dz * dz;
+                            amb = amb || fabs(dist - g.r2) <= mrg;
+                            if (!amb && dist < g.r2) {
+                                in = true;
+                                // low words of t hold the (non-negative) indices
+                                key = lattice_key((unsigned)__double2loint(tx), (unsigned)__double2loint(ty),
+                                                  (unsigned)__double2loint(tz), (unsigned)g.N);
                             }
                         }
                         // any decision within the error bound: redo it exactly from FP64
                         if (amb) in = exact_member(a.x64[i], a.y64[i], a.z64[i] + g.H, g, sh.x, sh.y, sh.z, key);
-                        if (in) yv = (double)fe[j] * g.denom_inv;
+                        if (in) v0 = rint_to_i64((double)fe[j] * g.yscale);
                     }
                 }
                 if (in) {
-                    long long v0, v1;
-                    Acc<MODE>::split(yv, v0, v1);
                     unsigned int h = slot_hash(key, kLog2Slots);
-                    int probe = 0;
-                    for (; probe < SLOTS; ++probe) {
-                        const unsigned long long prev = atomicCAS(&tkey[h], kEmpty, key);
-                        if (prev == kEmpty) {
-                            atomicAdd(&s_occ, 1);
-                            break;
-                        }
-                        if (prev == key) break;
+                    unsigned long long prev = 0;
+                    for (int probe = 0; probe < SLOTS; ++probe) {
+                        prev = atomicCAS(&tkey[h], kEmpty, key);
+                        if (prev == kEmpty || prev == key) break;
                         h = (h + 1) & (SLOTS - 1);
                     }
-                    if (probe == SLOTS) {
-                        atomicOr(a.error, 1);
-                    } else {
+                    if (prev == kEmpty || prev == key) {
+                        fresh += (prev == kEmpty);
                         atomicAdd(&tacc0[h], (unsigned long long)v0);
                         if (kWords == 2) atomicAdd(&tacc1[h], (unsigned long long)v1);
+                    } else {
+                        atomicOr(a.error, 1);
                     }
                 }
             }
+            count_fresh(fresh);
             __syncthreads();
             const int64_t next = c + (int64_t)kPPT * kBlk;
             if (s_occ >= SLOTS / 2 && next < p_end) {
@@ -476,7 +529,7 @@ __global__ __launch_bounds__(kBT) void batch_kernel(BatchArgs a)
             }
         }
         flush(kEmpty, seg_id);
-        if (a.hist_lds_words && s_binned > 0x40000000ull) {
+        if (a.hist_lds_words && binned > 0x40000000ll) {
             // keep the 32-bit LDS counters far from overflow
 #pragma unroll
             for (int h = 0; h < kMaxHists; ++h) {
@@ -491,8 +544,7 @@ __global__ __launch_bounds__(kBT) void batch_kernel(BatchArgs a)
                 }
             }
             __syncthreads();
-            if (tid == 0) s_binned = 0;
-            __syncthreads();
+            binned = 0;
         }
     }
     __syncthreads();
@@ -604,7 +656,8 @@ struct st_library {
     int64_t device_bytes = 0;
     double *x64 = nullptr, *y64 = nullptr, *z64 = nullptr, *e64 = nullptr;
     float *dx32 = nullptr, *dy32 = nullptr, *dz32 = nullptr, *e32 = nullptr;
-    double *zfirst = nullptr, *origin = nullptr, *eps = nullptr;
+    double *zfirst = nullptr;
+    BlockF32 *blocks = nullptr;
     TrackDesc *tracks_dev = nullptr;
     GeomDev *geoms_dev = nullptr;
 };
@@ -653,8 +706,7 @@ BatchArgs make_args(st_batch *b)
     a.dz32 = L->dz32;
     a.e32 = L->e32;
     a.zfirst = L->zfirst;
-    a.origin = L->origin;
-    a.eps = L->eps;
+    a.blocks = L->blocks;
     a.geoms = L->geoms_dev;
     a.order = b->order_dev;
     a.n_seg = (int64_t)L->fast_order.size() * b->opts.n_oversamples;
@@ -811,14 +863,13 @@ int st_library_create(const double *x, const double *y, const double *z, const d
     std::vector<double> sx(n_points), sy(n_points), sz(n_points), se(n_points);
     std::vector<double> zfirst(std::max<int64_t>(nblocks, 1));
     std::vector<float> fx, fy, fz, fe;
-    std::vector<double> origin, eps;
+    std::vector<BlockF32> blocks;
     if (precision == ST_F32) {
         fx.resize(n_points);
         fy.resize(n_points);
         fz.resize(n_points);
         fe.resize(n_points);
-        origin.resize(3 * std::max<int64_t>(nblocks, 1));
-        eps.resize(std::max<int64_t>(nblocks, 1));
+        blocks.resize(std::max<int64_t>(nblocks, 1));
     }
     const double *X = x, *Y = y, *Z = z, *E = edep;
     const int64_t base = offsets[0];
@@ -858,7 +909,8 @@ int st_library_create(const double *x, const double *y, const double *z, const d
         // fixed-point headroom: sum of |y| over the track
         const double ysum = esum / denom;
         const double ylim = (precision == ST_F64) ? 2.7e11 : 1.0e9;
-        bool fast = finite && n <= kMaxFastPoints && ysum < ylim && emax / denom < ylim;
+        const double ypoint = (precision == ST_F64) ? kMaxPointY64 : kMaxPointY32;
+        bool fast = finite && n <= kMaxFastPoints && ysum < ylim && emax / denom < ypoint;
         if (fast) {
             // entries kept by a flush come from the next point's layer (or, for
             // keys with an index equal to N, the layer below): a z-window two
@@ -881,7 +933,6 @@ int st_library_create(const double *x, const double *y, const double *z, const d
                 double org[3];
                 for (int a3 = 0; a3 < 3; ++a3) {
                     org[a3] = (std::isfinite(lo3[a3]) && std::isfinite(hi3[a3])) ? 0.5 * (lo3[a3] + hi3[a3]) : 0.0;
-                    origin[3 * (td.block0 + b) + a3] = org[a3];
                 }
                 double err = 0.0;
                 float *farr[3] = {fx.data(), fy.data(), fz.data()};
@@ -896,7 +947,16 @@ int st_library_create(const double *x, const double *y, const double *z, const d
                     }
                     fe[p] = (float)se[p];
                 }
-                eps[td.block0 + b] = err;
+                // dl: the FP32 rounding of the offsets plus, with a wide margin,
+                // the FP64 rounding of each add and product on the way to the
+                // sphere test (|shift| <= H' on each axis)
+                BlockF32 &bk = blocks[td.block0 + b];
+                bk.ox = org[0];
+                bk.oy = org[1];
+                bk.oz = org[2];
+                bk.dl = err + kTwoM44 * (std::fabs(org[0]) + std::fabs(org[1]) + std::fabs(org[2]) +
+                                         7.0 * g.half_length + std::fabs(g.gso) + g.diameter + 16.0);
+                if (!std::isfinite(bk.dl)) bk.dl = HUGE_VAL;
             }
         }
     });
@@ -919,6 +979,7 @@ int st_library_create(const double *x, const double *y, const double *z, const d
         gd[i].gso = g.gso;
         gd[i].denom = (2. / 3.) * g.diameter;
         gd[i].denom_inv = 1.0 / gd[i].denom;
+        gd[i].yscale = gd[i].denom_inv * kTwo32;
         gd[i].N = (unsigned long long)g.n_linear;
         gd[i].N2 = gd[i].N * gd[i].N;
     }
@@ -935,8 +996,7 @@ int st_library_create(const double *x, const double *y, const double *z, const d
         if (!rc) rc = dev_upload(&L->dy32, fy.data(), n_points, &bytes);
         if (!rc) rc = dev_upload(&L->dz32, fz.data(), n_points, &bytes);
         if (!rc) rc = dev_upload(&L->e32, fe.data(), n_points, &bytes);
-        if (!rc) rc = dev_upload(&L->origin, origin.data(), origin.size(), &bytes);
-        if (!rc) rc = dev_upload(&L->eps, eps.data(), eps.size(), &bytes);
+        if (!rc) rc = dev_upload(&L->blocks, blocks.data(), blocks.size(), &bytes);
     }
     if (!rc) rc = dev_upload(&L->tracks_dev, L->tracks.data(), L->tracks.size(), &bytes);
     if (!rc) rc = dev_upload(&L->geoms_dev, gd.data(), gd.size(), &bytes);
@@ -967,7 +1027,7 @@ void st_library_destroy(st_library *L)
     if (!L) return;
     (void)hipDeviceSynchronize();
     void *ptrs[] = {L->x64, L->y64, L->z64, L->e64, L->dx32, L->dy32, L->dz32, L->e32,
-                    L->zfirst, L->origin, L->eps, L->tracks_dev, L->geoms_dev};
+                    L->zfirst, L->blocks, L->tracks_dev, L->geoms_dev};
     for (void *p : ptrs) (void)hipFree(p);
     delete L;
 }

@@ -14,6 +14,7 @@
 
 #include "st_common.hpp"
 #include "st_internal.hpp"
+#include "st_device.hpp"
 
 namespace st {
 namespace {
@@ -274,7 +275,7 @@ __device__ __forceinline__ uint32_t block_exclusive_scan_1024(uint32_t v, uint32
 
 __device__ __forceinline__ void bin_sum(const HistArgs &a, double s)
 {
-    const int b = bin_of(a.edges, a.guess.nbins, s, bin_guess(a.guess, s));
+    const int b = bin_fast(a.edges, a.guess, s);
     if (b >= 0) atomicAdd(&a.bins[b], 1ull);
 }
 
@@ -988,6 +989,10 @@ int st_hist_create(int32_t type, float lo, float hi, int32_t nbins, st_hist **ou
     }
     if (!std::isfinite(h->guess.inv_width)) h->guess.inv_width = 0.0;
     if (!std::isfinite(h->guess.origin)) h->guess.origin = 0.0;
+    h->guess.log2_origin = (float)(h->guess.origin * 3.321928094887362);
+    h->guess.log2_inv_width = (float)(h->guess.inv_width / 3.321928094887362);
+    h->guess.e0 = h->edges_host[0];
+    h->guess.en = h->edges_host[(size_t)nbins];
     h->stream = default_stream();
     hipError_t e = hipMalloc((void **)&h->edges_dev, ((size_t)nbins + 1) * sizeof(double));
     if (e == hipSuccess) e = hipMalloc((void **)&h->bins_dev, (size_t)nbins * sizeof(uint64_t));

@@ -21,6 +21,9 @@ struct BinGuess {
     int nbins;
     double origin;   // lin: edges[0]; log: float log10(lo) widened
     double inv_width;
+    float log2_origin;      // log: origin * log2(10)
+    float log2_inv_width;   // log: inv_width / log2(10)
+    double e0, en;          // first and last edge
 };
 
 __host__ __device__ inline int bin_guess(const BinGuess &g, double v)
