@@ -9,7 +9,7 @@
 //     and the sphere layer index iz never decreases along the run);
 //   * FP64 SoA x, y, z, edep in that order (the data in ST_F64 mode, the exact
 //     side copy in ST_F32 mode);
-//   * ST_F32 mode: FP32 SoA dx, dy, dz, edep, with coordinates relative to a
+//   * ST_F32 mode: FP32 (dx, dy, dz, edep) per point, coordinates relative to a
 //     per-256-point block origin (FP64) and a per-block bound on the FP32
 //     rounding error;
 //   * per block: z of its first point (the search index); per track: offsets,
@@ -34,6 +34,7 @@
 #include <deque>
 #include <new>
 #include <numeric>
+#include <type_traits>
 #include <vector>
 
 #include "st_common.hpp"
@@ -101,7 +102,7 @@ struct BatchArgs {
     // library
     const TrackDesc *tracks;
     const double *x64, *y64, *z64, *e64;
-    const float *dx32, *dy32, *dz32, *e32;
+    const float4 *p32;          // ST_F32: (dx, dy, dz, edep) per point
     const double *zfirst;
     const BlockF32 *blocks;
     const GeomDev *geoms;
@@ -137,6 +138,20 @@ __device__ __forceinline__ long long uniform_i64(long long v)
     const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)v);
     const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)((unsigned long long)v >> 32));
     return (long long)(((unsigned long long)hi << 32) | lo);
+}
+
+// Block record through the scalar cache (read-only during the kernel, index
+// uniform across the workgroup).
+__device__ __forceinline__ BlockF32 load_block(const BlockF32 *p, int64_t idx)
+{
+    typedef const __attribute__((address_space(4))) double *ConstPtr;
+    const ConstPtr q = (ConstPtr)(const void *)(p + idx);
+    BlockF32 r;
+    r.ox = q[0];
+    r.oy = q[1];
+    r.oz = q[2];
+    r.dl = q[3];
+    return r;
 }
 
 // key = kx + ky * N + kz * N^2 (mod 2^64, as the reference's unsigned sum)
@@ -420,82 +435,76 @@ __global__ __launch_bounds__(kBT) void batch_kernel(BatchArgs a)
         const int b_lo = uniform_i32(s_range[0]), b_hi = uniform_i32(s_range[1]);
         if (b_lo >= b_hi) continue;
         const int64_t p_end = td.offset + min((int64_t)b_hi * kBlk, (int64_t)td.n);
+        const int64_t p_safe = td.offset + (int64_t)b_lo * kBlk;   // a point of the range
         binned += (long long)(b_hi - b_lo) * kBlk;
-        for (int b = b_lo; b < b_hi; b += kPPT) {
-            const int64_t c = td.offset + (int64_t)b * kBlk;
-            // loads of all kPPT points of this thread first, then the arithmetic
-            bool valid[kPPT];
-            double px[kPPT], py[kPPT], pz[kPPT], pe[kPPT];
-            float fx[kPPT], fy[kPPT], fz[kPPT], fe[kPPT];
-            BlockF32 bk[kPPT];
+        // Software pipeline: the points of the next iteration are loaded
+        // before the current ones are scored.  Lanes past the end of the
+        // range load a valid point and are masked off.
+        using PointT = typename std::conditional<MODE == ST_F64, double4, float4>::type;
+        PointT nxt[kPPT];
+        auto fetch = [&](int b, PointT *dst) {
 #pragma unroll
             for (int j = 0; j < kPPT; ++j) {
-                const int64_t i = c + (int64_t)j * kBlk + tid;
-                valid[j] = (b + j < b_hi) && i < p_end;
-                if constexpr (MODE == ST_F64) {
-                    if (valid[j]) {
-                        px[j] = a.x64[i];
-                        py[j] = a.y64[i];
-                        pz[j] = a.z64[i];
-                        pe[j] = a.e64[i];
-                    }
-                } else {
-                    if (valid[j]) {
-                        fx[j] = a.dx32[i];
-                        fy[j] = a.dy32[i];
-                        fz[j] = a.dz32[i];
-                        fe[j] = a.e32[i];
-                    }
-                    if (b + j < b_hi) bk[j] = a.blocks[td.block0 + b + j];
-                }
+                int64_t i = td.offset + (int64_t)(b + j) * kBlk + tid;
+                if (!(b + j < b_hi && i < p_end)) i = p_safe;
+                if constexpr (MODE == ST_F64) dst[j] = make_double4(a.x64[i], a.y64[i], a.z64[i], a.e64[i]);
+                else dst[j] = a.p32[i];
             }
+        };
+        fetch(b_lo, nxt);
+        for (int b = b_lo; b < b_hi; b += kPPT) {
+            PointT cur[kPPT];
+#pragma unroll
+            for (int j = 0; j < kPPT; ++j) cur[j] = nxt[j];
+            if (b + kPPT < b_hi) fetch(b + kPPT, nxt);
+            const int64_t c = td.offset + (int64_t)b * kBlk;
             int fresh = 0;
 #pragma unroll
             for (int j = 0; j < kPPT; ++j) {
+                if (b + j >= b_hi) break;
                 const int64_t i = c + (int64_t)j * kBlk + tid;
+                const bool valid = i < p_end;
                 bool in = false;
                 unsigned long long key = 0;
                 long long v0 = 0, v1 = 0;
-                if (valid[j]) {
-                    if constexpr (MODE == ST_F64) {
-                        in = exact_member(px[j], py[j], pz[j] + g.H, g, sh.x, sh.y, sh.z, key);
-                        if (in) Acc<ST_F64>::split(pe[j] / g.denom, v0, v1);
-                    } else {
-                        // block-uniform terms
-                        const double dl = bk[j].dl;
-                        const double cx = bk[j].ox + sh.x;
-                        const double cy = bk[j].oy + sh.y;
-                        const double cz = (bk[j].oz + g.H) + sh.z;
-                        const double h_in = g.H - dl, h_out = g.H + dl;
-                        const double fm = 0.5 - (dl * g.invD + 1e-12);
-                        const double mrg = 3.0 * dl * (g.D + 2.0 * dl) + 1e-14 * g.r2;
-                        // per point: approximate shifted coordinates, |error| <= dl
-                        const double xs = cx + (double)fx[j];
-                        const double ys = cy + (double)fy[j];
-                        const double zs = cz + (double)fz[j];
-                        const double axs = fabs(xs), ays = fabs(ys), azs = fabs(zs);
-                        const bool box_in = axs < h_in && ays < h_in && azs < h_in;
-                        bool amb = !box_in && !(axs > h_out || ays > h_out || azs > h_out);
-                        if (box_in) {
-                            const double ax = xs - g.gso, ay = ys - g.gso, az = zs - g.gso;
-                            const double qx = ax * g.invD, qy = ay * g.invD, qz = az * g.invD;
-                            const double tx = qx + kRoundMagic, ty = qy + kRoundMagic, tz = qz + kRoundMagic;
-                            const double kx = tx - kRoundMagic, ky = ty - kRoundMagic, kz = tz - kRoundMagic;
-                            amb = fabs(qx - kx) >= fm || fabs(qy - ky) >= fm || fabs(qz - kz) >= fm;
-                            const double dx = kx * g.D - ax, dy = ky * g.D - ay, dz = kz * g.D - az;
-                            const double dist = dx * dx + dy * dy + dz * dz;
-                            amb = amb || fabs(dist - g.r2) <= mrg;
-                            if (!amb && dist < g.r2) {
-                                in = true;
-                                // low words of t hold the (non-negative) indices
-                                key = lattice_key((unsigned)__double2loint(tx), (unsigned)__double2loint(ty),
-                                                  (unsigned)__double2loint(tz), (unsigned)g.N);
-                            }
-                        }
-                        // any decision within the error bound: redo it exactly from FP64
-                        if (amb) in = exact_member(a.x64[i], a.y64[i], a.z64[i] + g.H, g, sh.x, sh.y, sh.z, key);
-                        if (in) v0 = rint_to_i64((double)fe[j] * g.yscale);
+                if constexpr (MODE == ST_F64) {
+                    if (valid) {
+                        in = exact_member(cur[j].x, cur[j].y, cur[j].z + g.H, g, sh.x, sh.y, sh.z, key);
+                        if (in) Acc<ST_F64>::split(cur[j].w / g.denom, v0, v1);
                     }
+                } else {
+                    // block-uniform terms (scalar loads)
+                    const BlockF32 bk = load_block(a.blocks, td.block0 + b + j);
+                    const double dl = bk.dl;
+                    const double cx = bk.ox + sh.x;
+                    const double cy = bk.oy + sh.y;
+                    const double cz = (bk.oz + g.H) + sh.z;
+                    const double h_in = g.H - dl, h_out = g.H + dl;
+                    const double fm = 0.5 - (dl * g.invD + 1e-12);
+                    const double mrg = 3.0 * dl * (g.D + 2.0 * dl) + 1e-14 * g.r2;
+                    // per point: approximate shifted coordinates, |error| <= dl
+                    const double xs = cx + (double)cur[j].x;
+                    const double ys = cy + (double)cur[j].y;
+                    const double zs = cz + (double)cur[j].z;
+                    const double axs = fabs(xs), ays = fabs(ys), azs = fabs(zs);
+                    const bool box_in = axs < h_in && ays < h_in && azs < h_in;
+                    const bool box_out = axs > h_out || ays > h_out || azs > h_out;
+                    const double ax = xs - g.gso, ay = ys - g.gso, az = zs - g.gso;
+                    const double qx = ax * g.invD, qy = ay * g.invD, qz = az * g.invD;
+                    const double tx = qx + kRoundMagic, ty = qy + kRoundMagic, tz = qz + kRoundMagic;
+                    const double kx = tx - kRoundMagic, ky = ty - kRoundMagic, kz = tz - kRoundMagic;
+                    const double dx = kx * g.D - ax, dy = ky * g.D - ay, dz = kz * g.D - az;
+                    const double dist = dx * dx + dy * dy + dz * dz;
+                    const bool near = fabs(qx - kx) >= fm || fabs(qy - ky) >= fm || fabs(qz - kz) >= fm ||
+                                      fabs(dist - g.r2) <= mrg;
+                    // any decision within the error bound is redone exactly from FP64
+                    const bool amb = valid && !box_out && (!box_in || near);
+                    in = valid && box_in && !near && dist < g.r2;
+                    // low words of t hold the (non-negative) indices
+                    key = lattice_key((unsigned)__double2loint(tx), (unsigned)__double2loint(ty),
+                                      (unsigned)__double2loint(tz), (unsigned)g.N);
+                    if (amb) in = exact_member(a.x64[i], a.y64[i], a.z64[i] + g.H, g, sh.x, sh.y, sh.z, key);
+                    v0 = rint_to_i64((double)cur[j].w * g.yscale);
                 }
                 if (in) {
                     unsigned int h = slot_hash(key, kLog2Slots);
@@ -655,7 +664,7 @@ struct st_library {
     std::vector<int32_t> exact;        // tracks for the exact path
     int64_t device_bytes = 0;
     double *x64 = nullptr, *y64 = nullptr, *z64 = nullptr, *e64 = nullptr;
-    float *dx32 = nullptr, *dy32 = nullptr, *dz32 = nullptr, *e32 = nullptr;
+    float4 *p32 = nullptr;
     double *zfirst = nullptr;
     BlockF32 *blocks = nullptr;
     TrackDesc *tracks_dev = nullptr;
@@ -701,10 +710,7 @@ BatchArgs make_args(st_batch *b)
     a.y64 = L->y64;
     a.z64 = L->z64;
     a.e64 = L->e64;
-    a.dx32 = L->dx32;
-    a.dy32 = L->dy32;
-    a.dz32 = L->dz32;
-    a.e32 = L->e32;
+    a.p32 = L->p32;
     a.zfirst = L->zfirst;
     a.blocks = L->blocks;
     a.geoms = L->geoms_dev;
@@ -862,13 +868,10 @@ int st_library_create(const double *x, const double *y, const double *z, const d
 
     std::vector<double> sx(n_points), sy(n_points), sz(n_points), se(n_points);
     std::vector<double> zfirst(std::max<int64_t>(nblocks, 1));
-    std::vector<float> fx, fy, fz, fe;
+    std::vector<float4> p32;
     std::vector<BlockF32> blocks;
     if (precision == ST_F32) {
-        fx.resize(n_points);
-        fy.resize(n_points);
-        fz.resize(n_points);
-        fe.resize(n_points);
+        p32.resize(n_points);
         blocks.resize(std::max<int64_t>(nblocks, 1));
     }
     const double *X = x, *Y = y, *Z = z, *E = edep;
@@ -935,17 +938,16 @@ int st_library_create(const double *x, const double *y, const double *z, const d
                     org[a3] = (std::isfinite(lo3[a3]) && std::isfinite(hi3[a3])) ? 0.5 * (lo3[a3] + hi3[a3]) : 0.0;
                 }
                 double err = 0.0;
-                float *farr[3] = {fx.data(), fy.data(), fz.data()};
                 for (int64_t p = p0; p < p1; ++p) {
                     for (int a3 = 0; a3 < 3; ++a3) {
                         const double d = arr[a3][p] - org[a3];
                         const float f = (float)d;
-                        farr[a3][p] = f;
+                        (a3 == 0 ? p32[p].x : a3 == 1 ? p32[p].y : p32[p].z) = f;
                         const double e1 = std::fabs((double)f - d) + std::fabs(d) * 2.3e-16;
                         if (std::isfinite(e1)) err = std::max(err, e1);
                         else err = HUGE_VAL;
                     }
-                    fe[p] = (float)se[p];
+                    p32[p].w = (float)se[p];
                 }
                 // dl: the FP32 rounding of the offsets plus, with a wide margin,
                 // the FP64 rounding of each add and product on the way to the
@@ -992,10 +994,7 @@ int st_library_create(const double *x, const double *y, const double *z, const d
     if (!rc) rc = dev_upload(&L->e64, se.data(), n_points, &bytes);
     if (!rc) rc = dev_upload(&L->zfirst, zfirst.data(), zfirst.size(), &bytes);
     if (!rc && precision == ST_F32) {
-        rc = dev_upload(&L->dx32, fx.data(), n_points, &bytes);
-        if (!rc) rc = dev_upload(&L->dy32, fy.data(), n_points, &bytes);
-        if (!rc) rc = dev_upload(&L->dz32, fz.data(), n_points, &bytes);
-        if (!rc) rc = dev_upload(&L->e32, fe.data(), n_points, &bytes);
+        rc = dev_upload(&L->p32, p32.data(), n_points, &bytes);
         if (!rc) rc = dev_upload(&L->blocks, blocks.data(), blocks.size(), &bytes);
     }
     if (!rc) rc = dev_upload(&L->tracks_dev, L->tracks.data(), L->tracks.size(), &bytes);
@@ -1026,7 +1025,7 @@ void st_library_destroy(st_library *L)
 {
     if (!L) return;
     (void)hipDeviceSynchronize();
-    void *ptrs[] = {L->x64, L->y64, L->z64, L->e64, L->dx32, L->dy32, L->dz32, L->e32,
+    void *ptrs[] = {L->x64, L->y64, L->z64, L->e64, L->p32,
                     L->zfirst, L->blocks, L->tracks_dev, L->geoms_dev};
     for (void *p : ptrs) (void)hipFree(p);
     delete L;
