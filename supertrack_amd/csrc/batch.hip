@@ -229,11 +229,19 @@ struct Acc<ST_F32> {
     }
 };
 
-// 32-bit multiplicative hash of a 64-bit key
-__device__ __forceinline__ unsigned int slot_hash(unsigned long long key, int log2_slots)
+// Home slot in [0, SLOTS) of a 64-bit key: 32-bit multiplicative hash
+// scaled to the table size.
+template <int SLOTS>
+__device__ __forceinline__ unsigned int slot_hash(unsigned long long key)
 {
     const unsigned x = (unsigned)key ^ ((unsigned)(key >> 32) * 0x85EBCA6Bu);
-    return (x * 0x9E3779B1u) >> (32 - log2_slots);
+    return (unsigned)(((unsigned long long)(x * 0x9E3779B1u) * SLOTS) >> 32);
+}
+
+template <int SLOTS>
+__device__ __forceinline__ unsigned int next_slot(unsigned int h)
+{
+    return (h + 1 == (unsigned)SLOTS) ? 0u : h + 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -267,27 +275,42 @@ __device__ int wave_lower_bound(int n, Pred pred)
 // the persistent fused kernel
 // ---------------------------------------------------------------------------
 // kPPT: library blocks (points per thread) loaded together per iteration.
+// The FP32 kernel with the 1536-slot table (31 KB of LDS) is asked to fit five
+// workgroups per CU: 20 waves instead of 16 hide more of the LDS-atomic and
+// load latency.
 template <int MODE, int SLOTS, int kPPT>
-__global__ __launch_bounds__(kBT) void batch_kernel(BatchArgs a)
+constexpr int waves_per_eu()
 {
-    static_assert(SLOTS == 2048 || SLOTS == 4096, "table sizes");
+    return (MODE == ST_F32 && SLOTS == 1536 && kPPT == 1) ? 5 : 1;
+}
+
+template <int MODE, int SLOTS, int kPPT>
+__global__ __launch_bounds__(kBT) __attribute__((amdgpu_waves_per_eu(waves_per_eu<MODE, SLOTS, kPPT>())))
+void batch_kernel(BatchArgs a)
+{
+    static_assert(SLOTS == 1536 || SLOTS == 2048 || SLOTS == 4096, "table sizes");
     // occupancy never exceeds max(half table, survivors) + one iteration's points
     static_assert((SLOTS / 2 > kMaxWindow ? SLOTS / 2 : kMaxWindow) + kPPT * kBlk <= SLOTS * 9 / 10,
                   "the table must hold the survivors of a flush plus an iteration");
-    constexpr int kLog2Slots = (SLOTS == 2048) ? 11 : 12;
-    constexpr int kPer = SLOTS / kBT;            // table slots scanned per thread
+    // entries at a flush: those kept by the previous flush (at most kMaxWindow)
+    // or below SLOTS / 2, plus the points of the last iteration
+    constexpr int kMaxEntries = (SLOTS / 2 - 1 > kMaxWindow ? SLOTS / 2 - 1 : (int)kMaxWindow) + kPPT * kBlk;
+    constexpr int kLP = (kMaxEntries + kBT - 1) / kBT;   // list entries per thread
     constexpr int kWords = Acc<MODE>::words;
     extern __shared__ unsigned long long lds[];
     unsigned long long *tkey = lds;
     unsigned long long *tacc0 = tkey + SLOTS;
     unsigned long long *tacc1 = tacc0 + SLOTS;  // used when kWords == 2
-    unsigned int *lhist = reinterpret_cast<unsigned int *>(tacc0 + SLOTS * kWords);
+    // slots in use, in insertion order (s_occ entries)
+    unsigned short *tlist = reinterpret_cast<unsigned short *>(tacc0 + SLOTS * kWords);
+    unsigned int *lhist = reinterpret_cast<unsigned int *>(tlist + SLOTS);
 
     __shared__ long long s_seg;
     __shared__ int s_occ;
     __shared__ int s_range[2];
 
     const int tid = threadIdx.x;
+    const int lane = tid & 63;
     for (int s = tid; s < SLOTS; s += kBT) {
         tkey[s] = kEmpty;
         tacc0[s] = 0;
@@ -299,51 +322,76 @@ __global__ __launch_bounds__(kBT) void batch_kernel(BatchArgs a)
     // upper bound on the counts added to any LDS bin since the last spill
     long long binned = 0;
 
-    // adds the wave's count of new table entries to the occupancy
-    auto count_fresh = [&](int n) {
-        int wsum = n;
+    // Appends the slots that lanes newly occupied (slot >= 0) to the list:
+    // one LDS atomic per wave reserves the positions.
+    auto append = [&](const int *slot, auto count_tag) {
+        constexpr int kN = decltype(count_tag)::value;
+        unsigned long long m[kN];
+        int cnt = 0;
 #pragma unroll
-        for (int d = 32; d >= 1; d >>= 1) wsum += __shfl_xor(wsum, d, 64);
-        if ((tid & 63) == 0 && wsum) atomicAdd(&s_occ, wsum);
+        for (int j = 0; j < kN; ++j) {
+            m[j] = __ballot(slot[j] >= 0);
+            cnt += __popcll(m[j]);
+        }
+        if (cnt == 0) return;
+        int base = 0;
+        if (lane == 0) base = atomicAdd(&s_occ, cnt);
+        base = __shfl(base, 0, 64);
+#pragma unroll
+        for (int j = 0; j < kN; ++j) {
+            const int below = __builtin_amdgcn_mbcnt_hi((unsigned)(m[j] >> 32),
+                                                        __builtin_amdgcn_mbcnt_lo((unsigned)m[j], 0u));
+            if (slot[j] >= 0 && base + below < SLOTS) tlist[base + below] = (unsigned short)slot[j];
+            base += __popcll(m[j]);
+        }
     };
 
     // Flushes entries with key < bound (all entries when bound == kEmpty):
     // complete sums are converted to lineal energy and binned; the rest are
-    // re-inserted into the cleared table.  The bin edges of a thread's slots
-    // are loaded together before any is compared, so the global-load latency
-    // is paid once per group of slots rather than once per sphere.
+    // re-inserted into the cleared table.  Threads walk the dense list of
+    // used slots, so the work follows the entries, not the table size, and
+    // the bin edges of two entries are loaded before either is compared.
     auto flush = [&](unsigned long long bound, int64_t seg) {
         __syncthreads();
-        unsigned long long kk[kPer], k0[kPer], k1[kPer];
-        constexpr int kGroup = 2;
+        int n = uniform_i32(s_occ);
+        if (n > kLP * kBT) {   // cannot happen (the bound above); reported, never read past the list
+            if (tid == 0) atomicOr(a.error, 2);
+            n = kLP * kBT < SLOTS ? kLP * kBT : SLOTS;
+        }
+        unsigned long long kk[kLP], k0[kLP], k1[kLP];
 #pragma unroll
-        for (int j0 = 0; j0 < kPer; j0 += kGroup) {
-            double vv[kGroup];
-            bool em[kGroup];
+        for (int r = 0; r < kLP; ++r) kk[r] = kEmpty;
 #pragma unroll
-            for (int u = 0; u < kGroup; ++u) {
-                const int j = j0 + u;
-                const int s = tid + j * kBT;
-                const unsigned long long k = tkey[s];
-                const unsigned long long v0 = tacc0[s];
-                const unsigned long long v1 = (kWords == 2) ? tacc1[s] : 0ull;
-                const bool full = k != kEmpty;
-                em[u] = full && (bound == kEmpty || k < bound);
-                kk[j] = (full && !em[u]) ? k : kEmpty;
-                k0[j] = v0;
-                k1[j] = v1;
-                vv[u] = Acc<MODE>::value(v0, v1);
-                if (full) {
-                    tkey[s] = kEmpty;
-                    tacc0[s] = 0;
-                    if (kWords == 2) tacc1[s] = 0;
-                }
-                if (em[u] && a.rec_cap > 0) {
-                    const unsigned long long p = atomicAdd(a.rec_count, 1ull);
-                    if ((int64_t)p < a.rec_cap) {
-                        a.rec_seg[p] = seg;
-                        a.rec_key[p] = k;
-                        a.rec_sum[p] = vv[u];
+        for (int r0 = 0; r0 < kLP; r0 += 2) {
+            double vv[2];
+            bool em[2];
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const int r = r0 + u;
+                em[u] = false;
+                vv[u] = 0.0;
+                if (r >= kLP) continue;
+                const int idx = tid + r * kBT;
+                if (idx < n) {
+                    const int sl = min((int)tlist[idx], SLOTS - 1);
+                    const unsigned long long k = tkey[sl];
+                    const unsigned long long v0 = tacc0[sl];
+                    const unsigned long long v1 = (kWords == 2) ? tacc1[sl] : 0ull;
+                    em[u] = k < bound;
+                    if (!em[u]) kk[r] = k;
+                    k0[r] = v0;
+                    k1[r] = v1;
+                    vv[u] = Acc<MODE>::value(v0, v1);
+                    tkey[sl] = kEmpty;
+                    tacc0[sl] = 0;
+                    if (kWords == 2) tacc1[sl] = 0;
+                    if (em[u] && a.rec_cap > 0) {
+                        const unsigned long long p = atomicAdd(a.rec_count, 1ull);
+                        if ((int64_t)p < a.rec_cap) {
+                            a.rec_seg[p] = seg;
+                            a.rec_key[p] = k;
+                            a.rec_sum[p] = vv[u];
+                        }
                     }
                 }
             }
@@ -351,10 +399,10 @@ __global__ __launch_bounds__(kBT) void batch_kernel(BatchArgs a)
             for (int h = 0; h < kMaxHists; ++h) {
                 if (h >= a.n_hists) break;
                 const HistDev &hd = a.hist[h];
-                int bb[kGroup];
-                double lo[kGroup], hi[kGroup];
+                int bb[2];
+                double lo[2], hi[2];
 #pragma unroll
-                for (int u = 0; u < kGroup; ++u) {
+                for (int u = 0; u < 2; ++u) {
                     // closed-form guess, clamped into [0, nbins - 1]: always a valid load
                     const double v = vv[u];
                     float t;
@@ -365,7 +413,7 @@ __global__ __launch_bounds__(kBT) void batch_kernel(BatchArgs a)
                     hi[u] = hd.edges[bb[u] + 1];
                 }
 #pragma unroll
-                for (int u = 0; u < kGroup; ++u) {
+                for (int u = 0; u < 2; ++u) {
                     if (!em[u]) continue;
                     const double v = vv[u];
                     int b = bb[u];
@@ -380,22 +428,23 @@ __global__ __launch_bounds__(kBT) void batch_kernel(BatchArgs a)
         __syncthreads();
         if (tid == 0) s_occ = 0;
         __syncthreads();
-        int fresh = 0;
+        int placed[kLP];
 #pragma unroll
-        for (int j = 0; j < kPer; ++j) {
-            if (kk[j] != kEmpty) {
-                unsigned int h = slot_hash(kk[j], kLog2Slots);
+        for (int r = 0; r < kLP; ++r) {
+            placed[r] = -1;
+            if (kk[r] != kEmpty) {
+                unsigned int h = slot_hash<SLOTS>(kk[r]);
                 for (int probe = 0; probe < SLOTS; ++probe) {
-                    const unsigned long long prev = atomicCAS(&tkey[h], kEmpty, kk[j]);
+                    const unsigned long long prev = atomicCAS(&tkey[h], kEmpty, kk[r]);
                     if (prev == kEmpty) break;
-                    h = (h + 1) & (SLOTS - 1);
+                    h = next_slot<SLOTS>(h);
                 }
-                ++fresh;
-                atomicAdd(&tacc0[h], k0[j]);
-                if (kWords == 2) atomicAdd(&tacc1[h], k1[j]);
+                placed[r] = (int)h;
+                atomicAdd(&tacc0[h], k0[r]);
+                if (kWords == 2) atomicAdd(&tacc1[h], k1[r]);
             }
         }
-        count_fresh(fresh);
+        append(placed, std::integral_constant<int, kLP>());
         __syncthreads();
     };
 
@@ -458,7 +507,9 @@ __global__ __launch_bounds__(kBT) void batch_kernel(BatchArgs a)
             for (int j = 0; j < kPPT; ++j) cur[j] = nxt[j];
             if (b + kPPT < b_hi) fetch(b + kPPT, nxt);
             const int64_t c = td.offset + (int64_t)b * kBlk;
-            int fresh = 0;
+            int fresh[kPPT];
+#pragma unroll
+            for (int j = 0; j < kPPT; ++j) fresh[j] = -1;
 #pragma unroll
             for (int j = 0; j < kPPT; ++j) {
                 if (b + j >= b_hi) break;
@@ -507,15 +558,15 @@ __global__ __launch_bounds__(kBT) void batch_kernel(BatchArgs a)
                     v0 = rint_to_i64((double)cur[j].w * g.yscale);
                 }
                 if (in) {
-                    unsigned int h = slot_hash(key, kLog2Slots);
+                    unsigned int h = slot_hash<SLOTS>(key);
                     unsigned long long prev = 0;
                     for (int probe = 0; probe < SLOTS; ++probe) {
                         prev = atomicCAS(&tkey[h], kEmpty, key);
                         if (prev == kEmpty || prev == key) break;
-                        h = (h + 1) & (SLOTS - 1);
+                        h = next_slot<SLOTS>(h);
                     }
                     if (prev == kEmpty || prev == key) {
-                        fresh += (prev == kEmpty);
+                        if (prev == kEmpty) fresh[j] = (int)h;
                         atomicAdd(&tacc0[h], (unsigned long long)v0);
                         if (kWords == 2) atomicAdd(&tacc1[h], (unsigned long long)v1);
                     } else {
@@ -523,7 +574,7 @@ __global__ __launch_bounds__(kBT) void batch_kernel(BatchArgs a)
                     }
                 }
             }
-            count_fresh(fresh);
+            append(fresh, std::integral_constant<int, kPPT>());
             __syncthreads();
             const int64_t next = c + (int64_t)kPPT * kBlk;
             if (s_occ >= SLOTS / 2 && next < p_end) {
@@ -584,20 +635,23 @@ const void *kernel_ptr()
     return (const void *)batch_kernel<MODE, SLOTS, PPT>;
 }
 
+template <int MODE>
+const void *pick_kernel_mode(int slots, int ppt)
+{
+    if (slots == 1536) return ppt == 2 ? kernel_ptr<MODE, 1536, 2>() : kernel_ptr<MODE, 1536, 1>();
+    if (slots == 2048) return ppt == 2 ? kernel_ptr<MODE, 2048, 2>() : kernel_ptr<MODE, 2048, 1>();
+    return ppt == 2 ? kernel_ptr<MODE, 4096, 2>() : kernel_ptr<MODE, 4096, 1>();
+}
+
 const void *pick_kernel(int mode, int slots, int ppt)
 {
-    if (mode == ST_F64) {
-        if (slots == 2048) return ppt == 2 ? kernel_ptr<ST_F64, 2048, 2>() : kernel_ptr<ST_F64, 2048, 1>();
-        return ppt == 2 ? kernel_ptr<ST_F64, 4096, 2>() : kernel_ptr<ST_F64, 4096, 1>();
-    }
-    if (slots == 2048) return ppt == 2 ? kernel_ptr<ST_F32, 2048, 2>() : kernel_ptr<ST_F32, 2048, 1>();
-    return ppt == 2 ? kernel_ptr<ST_F32, 4096, 2>() : kernel_ptr<ST_F32, 4096, 1>();
+    return mode == ST_F64 ? pick_kernel_mode<ST_F64>(slots, ppt) : pick_kernel_mode<ST_F32>(slots, ppt);
 }
 
 size_t lds_bytes(int mode, int slots, int hist_words)
 {
     const int words = (mode == ST_F64) ? 2 : 1;
-    return (size_t)slots * 8 * (1 + words) + (size_t)hist_words * 4;
+    return (size_t)slots * (8 * (1 + words) + 2) + (size_t)hist_words * 4;
 }
 
 template <typename T>
@@ -1070,20 +1124,22 @@ int st_batch_create(st_library *lib, const st_batch_opts *opts, st_hist *const *
         st_vcs_set_stream(m, b->stream);
         b->methods.push_back(m);
     }
-    // 2048 slots hold the largest survivor set (kMaxWindow) plus a half table
-    // plus a chunk, so every fast-path track fits for every choice
-    int slots = opts->table_slots ? opts->table_slots : 2048;
-    if (slots != 2048 && slots != 4096)
-        return fail(set_error(ST_EINVAL, "st_batch_create: table_slots must be 2048 or 4096"));
-    int bpc = opts->blocks_per_cu ? opts->blocks_per_cu : 4;
-    const int ppt = opts->points_per_thread ? opts->points_per_thread : 1;
+    // every table size holds the largest survivor set (kMaxWindow) or a half
+    // table, plus an iteration's points, so every fast-path track fits
+    // defaults: the fastest shapes measured on MI355X for 2 nm and 10 nm spheres
+    const bool f32 = lib->precision == ST_F32;
+    int slots = opts->table_slots ? opts->table_slots : 1536;
+    if (slots != 1536 && slots != 2048 && slots != 4096)
+        return fail(set_error(ST_EINVAL, "st_batch_create: table_slots must be 1536, 2048 or 4096"));
+    int bpc = opts->blocks_per_cu ? opts->blocks_per_cu : (f32 ? 5 : 3);
+    const int ppt = opts->points_per_thread ? opts->points_per_thread : (f32 ? 1 : 2);
     if (ppt != 1 && ppt != 2) return fail(set_error(ST_EINVAL, "st_batch_create: points_per_thread must be 1 or 2"));
     rc = configure(b, slots, bpc, ppt);
     if (rc) return fail(rc);
     if (opts->autotune && !lib->fast_order.empty() && opts->n_oversamples > 0) {
         // time each candidate on one full pass; keep the fastest
-        const int cand_slots[] = {2048, 4096};
-        const int cand_bpc[] = {2, 3, 4, 6, 8};
+        const int cand_slots[] = {1536, 2048, 4096};
+        const int cand_bpc[] = {2, 3, 4, 5, 6, 8};
         hipEvent_t e0, e1;
         ST_HIP_TRY(hipEventCreate(&e0));
         ST_HIP_TRY(hipEventCreate(&e1));
@@ -1191,7 +1247,7 @@ int st_batch_sync(st_batch *b)
     ST_HIP_TRY(hipStreamSynchronize(b->stream));
     int err = 0;
     ST_HIP_TRY(hipMemcpy(&err, b->error, sizeof err, hipMemcpyDeviceToHost));
-    if (err) return set_error(ST_ESTATE, "batched kernel reported a full sphere table (code %d)", err);
+    if (err) return set_error(ST_ESTATE, "batched kernel reported a full sphere table or entry list (code %d)", err);
     return ST_OK;
 }
 
