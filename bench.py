@@ -212,6 +212,44 @@ def copy_probe():
     b.free()
 
 
+def plain_main(args, w, n_tracks, n_os):
+    """Plain FP64 drop-in path (cfg1, cfg2): every track, one launch sequence per track."""
+    import supertrack_amd as st
+    from supertrack_amd import workloads as W
+    host = W.make_library(w, n_tracks=n_tracks)
+    total_points = int(host[4][-1])
+    deposits = total_points * n_os
+    for _ in range(args.warmup):
+        time_plain(w, host, 1, n_os)
+    times = []
+    for _ in range(args.steps):
+        d, secs, nt, counts = time_plain(w, host, 1, n_os)
+        times.append(secs)
+    secs = float(np.mean(times))
+    value = deposits / secs
+    log(f"{args.workload} plain: {secs * 1e3:.1f} ms/step, {value:.3e} deposits/s")
+    cpu = None
+    if not args.no_cpu:
+        rate, threads, ns, cdep, csecs = cpu_baseline(w, 1, n_os)
+        cpu = {"value": rate, "unit": "deposits/s", "cores": threads, "kind": "port",
+               "sample": f"{args.workload}: all {ns} tracks x {n_os} oversamples, {cdep:.3e} deposits in "
+                         f"{csecs:.2f}s, FP64 oracle, OpenMP"}
+    out = {
+        "metric": "deposits scored/s on 1 MI355X, plain FP64 per-track path",
+        "value": value, "unit": "deposits/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": secs * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{args.workload}: {n_tracks} tracks, {total_points} points, {n_os} oversamples, "
+                               f"geometries {w['geometries']}, histograms {w['hists'][:1]}",
+                   "deposits_per_step": deposits, "parallelism": "1 GPU"},
+        "roofline": {"bound": "hbm", "achieved": deposits * 32 / secs / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": deposits * 32 / secs / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                     "basis": "32 B per deposit / host-clock step time (per-track launch sequences)"},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -228,6 +266,9 @@ def main():
     w = W.workload(args.workload)
     n_tracks = args.tracks or w["n_tracks"]
     n_os = args.oversamples or w["n_oversamples"]
+    if w.get("path") == "plain":
+        plain_main(args, w, n_tracks, n_os)
+        return
     lib, host = build_library(w, n_tracks, w["precision"])
     info = lib.info()
     total_points = info["n_points"]
@@ -295,7 +336,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{args.workload}: {n_tracks} tracks, {total_points} points, {n_os} oversamples, "
                                f"geometries {w['geometries']}, histograms {w['hists']}",
-                   "deposits_per_step": deposits, "parallelism": "1 GPU", "launch": params},
+                   "deposits_per_step": deposits, "parallelism": "1 GPU", "launch": params,
+                   "fast_tracks": info["fast_tracks"], "exact_tracks": info["exact_tracks"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "basis": f"{bytes_per_dep} B per deposit x {deposits} deposits / batch kernel time "

@@ -225,16 +225,26 @@ __global__ __launch_bounds__(kScoreBlock) void score_kernel(ScoreArgs a)
 // ---------------------------------------------------------------------------
 // Histogram: stable LSD radix sort -> left-to-right reduce-by-key -> bin
 // ---------------------------------------------------------------------------
-// Small inputs (m <= kLdsCap pairs) are sorted by one 1024-thread workgroup in
-// LDS with 4-bit digits; larger inputs by the onesweep kernels below with
-// 8-bit digits.  Only as many bits are sorted as the keys can have.
+// Small inputs (m <= kLdsCap pairs) are grouped by one 1024-thread workgroup
+// in LDS: a stable counting sort on a 12-bit hash of the key (three 4-bit
+// passes, whatever the key width) puts every key's pairs in one bucket, in
+// pair order; the reduce then sums each key's pairs left to right within its
+// bucket.  Larger inputs take the onesweep kernels below (8-bit digits over
+// the full keys).
 constexpr int kSortThreads = 1024;
 constexpr int kSortWaves = kSortThreads / 64;
-constexpr int kLdsCap = 6144;
+constexpr int kLdsCap = 12288;
 constexpr int kDigitBits = 4;
 constexpr int kDigits = 1 << kDigitBits;
-constexpr size_t kLdsBytes = (size_t)kLdsCap * (2 * sizeof(uint64_t) + 2 * sizeof(uint16_t)) +
+constexpr int kHashBits = 12;
+constexpr int kHashPasses = kHashBits / kDigitBits;
+constexpr size_t kLdsBytes = (size_t)kLdsCap * 4 * sizeof(uint16_t) +
                              (size_t)kDigits * kSortThreads * sizeof(uint16_t);
+
+__device__ __forceinline__ uint16_t key_bucket(uint64_t k)
+{
+    return (uint16_t)((k * 0x9E3779B97F4A7C15ull) >> (64 - kHashBits));
+}
 struct HistArgs {
     const uint64_t *keys;
     const double *y;
@@ -279,43 +289,33 @@ __device__ __forceinline__ void bin_sum(const HistArgs &a, double s)
     if (b >= 0) atomicAdd(&a.bins[b], 1ull);
 }
 
-// One workgroup: sort m <= kLdsCap pairs in LDS, reduce, bin.
+// One workgroup: group m <= kLdsCap pairs by key bucket in LDS, reduce, bin.
 __device__ void lds_sort_reduce_bin(const HistArgs &a, int m, unsigned char *smem)
 {
-    uint64_t *k0 = reinterpret_cast<uint64_t *>(smem);
-    uint64_t *k1 = k0 + kLdsCap;
-    uint16_t *v0 = reinterpret_cast<uint16_t *>(k1 + kLdsCap);
+    uint16_t *h0 = reinterpret_cast<uint16_t *>(smem);
+    uint16_t *h1 = h0 + kLdsCap;
+    uint16_t *v0 = h1 + kLdsCap;
     uint16_t *v1 = v0 + kLdsCap;
     uint16_t *cnt = v1 + kLdsCap;   // [kDigits][kSortThreads]
     __shared__ uint32_t s_wave[kSortWaves];
-    __shared__ unsigned long long s_max;
     const int tid = threadIdx.x;
-    if (tid == 0) s_max = 0;
-    __syncthreads();
-    unsigned long long my_max = 0;
     for (int i = tid; i < m; i += kSortThreads) {
-        const uint64_t kk = a.keys[i];
-        k0[i] = kk;
+        h0[i] = key_bucket(a.keys[i]);
         v0[i] = (uint16_t)i;
-        my_max = kk > my_max ? kk : my_max;
     }
-    atomicMax(&s_max, my_max);
     __syncthreads();
-    const unsigned long long kmax = s_max;
-    const int bits = kmax ? 64 - __clzll((long long)kmax) : 0;
-    const int passes = (bits + kDigitBits - 1) / kDigitBits;
     const int ipt = (m + kSortThreads - 1) / kSortThreads;
     const int my_lo = min(tid * ipt, m);
     const int my_hi = min(my_lo + ipt, m);
-    uint64_t *src_k = k0, *dst_k = k1;
+    uint16_t *src_h = h0, *dst_h = h1;
     uint16_t *src_v = v0, *dst_v = v1;
-    for (int p = 0; p < passes; ++p) {
+    for (int p = 0; p < kHashPasses; ++p) {
         const int shift = p * kDigitBits;
         uint32_t c[kDigits];
 #pragma unroll
         for (int d = 0; d < kDigits; ++d) c[d] = 0;
         for (int i = my_lo; i < my_hi; ++i) {
-            const int d = (int)((src_k[i] >> shift) & (kDigits - 1));
+            const int d = (src_h[i] >> shift) & (kDigits - 1);
 #pragma unroll
             for (int dd = 0; dd < kDigits; ++dd) c[dd] += (d == dd);
         }
@@ -336,23 +336,38 @@ __device__ void lds_sort_reduce_bin(const HistArgs &a, int m, unsigned char *sme
         for (int j = 0; j < kDigits; ++j) cnt[tid * kDigits + j] = (uint16_t)(base + local[j]);
         __syncthreads();
         for (int i = my_lo; i < my_hi; ++i) {
-            const uint64_t kk = src_k[i];
-            const int d = (int)((kk >> shift) & (kDigits - 1));
+            const uint16_t hh = src_h[i];
+            const int d = (hh >> shift) & (kDigits - 1);
             const uint32_t pos = cnt[d * kSortThreads + tid];
             cnt[d * kSortThreads + tid] = (uint16_t)(pos + 1);
-            dst_k[pos] = kk;
+            dst_h[pos] = hh;
             dst_v[pos] = src_v[i];
         }
         __syncthreads();
-        uint64_t *tk = src_k; src_k = dst_k; dst_k = tk;
+        uint16_t *th = src_h; src_h = dst_h; dst_h = th;
         uint16_t *tv = src_v; src_v = dst_v; dst_v = tv;
     }
-    // reduce-by-key left to right (Histogram.cu:153), bin (:158)
+    // reduce-by-key left to right (Histogram.cu:153), bin (:158): the pair at
+    // sorted position i starts its key's sum if no earlier pair of its bucket
+    // has the same key; the sum then takes the bucket's pairs of that key in
+    // order (ascending pair index: the sort is stable)
     for (int i = tid; i < m; i += kSortThreads) {
-        const uint64_t kk = src_k[i];
-        if (i > 0 && src_k[i - 1] == kk) continue;
-        double s = a.y[src_v[i]];
-        for (int j = i + 1; j < m && src_k[j] == kk; ++j) s = s + a.y[src_v[j]];
+        const uint16_t hb = src_h[i];
+        const int pi = src_v[i];
+        const uint64_t kk = a.keys[pi];
+        bool first = true;
+        for (int j = i - 1; j >= 0 && src_h[j] == hb; --j) {
+            if (a.keys[src_v[j]] == kk) {
+                first = false;
+                break;
+            }
+        }
+        if (!first) continue;
+        double s = a.y[pi];
+        for (int j = i + 1; j < m && src_h[j] == hb; ++j) {
+            const int pj = src_v[j];
+            if (a.keys[pj] == kk) s = s + a.y[pj];
+        }
         bin_sum(a, s);
     }
 }

@@ -35,11 +35,12 @@ LOG1000 = ("log", 1000, 1e-2, 1e3)
 WORKLOADS = {
     "cfg1": dict(
         n_tracks=1, geometries=[(2.5e3, 10.0)], chunk=0, hists=[LIN3000], n_oversamples=10,
-        synth=dict(_BASE_SYNTH, stream_id=1), precision=ST_F64,
+        synth=dict(_BASE_SYNTH, stream_id=1), precision=ST_F64, path="plain",
     ),
     "cfg2": dict(
         n_tracks=256, geometries=[(2.5e3, 10.0)], chunk=0, hists=[LIN3000], n_oversamples=100,
         synth=dict(_BASE_SYNTH, length_min_nm=5000.0, length_max_nm=20000.0, stream_id=2), precision=ST_F64,
+        path="plain",
     ),
     "cfg3": dict(
         n_tracks=4096, geometries=[(2.5e3, 10.0)], chunk=0, hists=[LOG1000], n_oversamples=100,

@@ -140,7 +140,8 @@ def test_histogram_tests_known_answers(capacity):
                                                  (250000, 1 << 27, None), (300000, 1 << 60, None),
                                                  (4000, 1 << 27, 100000), (0, 10, 100000),
                                                  (100000, 1 << 35, 200000), (131072, 1 << 20, None),
-                                                 (20000, 3, None)])
+                                                 (20000, 3, None), (12288, 1 << 20, None), (12289, 1 << 40, None),
+                                                 (12000, 1, None), (9000, 1 << 62, 12288)])
 def test_histogram_random_pairs_match_oracle(m, keyspace, capacity):
     rng = np.random.default_rng(m + 3)
     keys = rng.integers(0, keyspace, m, dtype=np.uint64)
