@@ -216,3 +216,46 @@ def test_batch_reference_known_answers(name):
         counts, _, info, edges = gpu_run(X, Y, Z, E, off, [F.geometry_of(name)], None, nos, 1234, False, False,
                                          precision, [(kind, nb, lo, hi)])
         np.testing.assert_array_equal(counts[0].astype(np.float64), F.expected_bins(edges[0], value, count))
+
+
+@pytest.mark.parametrize("precision,diameter", [(st.ST_F32, 10.0), (st.ST_F32, 2.0), (st.ST_F64, 10.0)])
+def test_batch_points_on_decision_boundaries(precision, diameter):
+    """Points placed within a few ulps of voxel faces, half-way planes and
+    sphere surfaces *after* each oversample's shift: the FP32 fast path must
+    route every such decision to the exact FP64 recheck, so keys equal the
+    oracle's exactly."""
+    rng = np.random.default_rng(int(diameter) + precision)
+    g = po.geometry(2.5e3, diameter)
+    H, D, r = g.half_length, g.diameter, g.radius
+    n_tracks, n_os, per_os = 4, 3, 6000
+    xs, ys, zs, es, off = [], [], [], [], [0]
+    for t in range(n_tracks):
+        sh = po.shift_table(W.SEED, t, n_os, True, True, g)
+        px, py, pz = [], [], []
+        for os in range(n_os):
+            sx, sy, sz = sh[os]
+            m = per_os
+            k = rng.integers(0, g.n_linear, (3, m))
+            base = g.gso + D * k                       # sphere centres (shifted frame)
+            kind = rng.integers(0, 4, (3, m))          # 0 centre-ish, 1 half-way, 2 +surface, 3 -surface
+            pos = np.where(kind == 1, base + 0.5 * D, np.where(kind == 2, base + r, np.where(kind == 3, base - r,
+                                                                                           base + rng.normal(0, 0.2 * D, (3, m)))))
+            face = rng.random((3, m)) < 0.05           # some on the voxel faces
+            pos = np.where(face, np.where(rng.random((3, m)) < 0.5, -H, H), pos)
+            pos = pos + rng.integers(-3, 4, (3, m)) * np.spacing(np.abs(pos) + 1.0)
+            # back to the unshifted frame (z also carries the +H' of the z shift)
+            px.append(pos[0] - sx)
+            py.append(pos[1] - sy)
+            pz.append(pos[2] - sz - H)
+        x = np.concatenate(px); y = np.concatenate(py); z = np.concatenate(pz)
+        xs.append(x); ys.append(y); zs.append(z); es.append(rng.uniform(1, 1000, len(x)))
+        off.append(off[-1] + len(x))
+    x, y, z, e = (np.concatenate(a) for a in (xs, ys, zs, es))
+    off = np.array(off, np.int64)
+    oracle = oracle_segments(x, y, z, e, off, [g], None, n_os, W.SEED, True, True)
+    counts, rec, info, edges = gpu_run(x, y, z, e, off, [(2.5e3, diameter)], None, n_os, W.SEED, True, True,
+                                       precision, [("log", 1000, 1e-2, 1e3)], record=400000)
+    assert info["exact_tracks"] == 0
+    n, _ = compare_records(rec, oracle, 1e-6 if precision == st.ST_F32 else 1e-12)
+    assert n > 1000
+    check_bins(counts, edges, oracle, 1e-6 if precision == st.ST_F32 else 1e-12)
