@@ -57,6 +57,8 @@ def parse():
     p.add_argument("--cpu-oversamples", type=int, default=0, help="CPU sample oversamples (0: workload's)")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--autotune", action="store_true")
+    p.add_argument("--workers", type=int, default=4,
+                   help="plain path: concurrent workers, one HIP stream each (the reference's [Run] Threads)")
     p.add_argument("--copy-probe", action="store_true",
                    help="only run a 1 GiB device-to-device copy 5 times (calibrates HBM byte counters)")
     return p.parse_args()
@@ -133,16 +135,24 @@ def time_batch(lib, w, n_os, steps, warmup, autotune, ev):
     return secs, float(np.mean(kernel_ms)), params, counts
 
 
-def time_plain(w, host, stride, n_os):
-    """Plain FP64 drop-in path (per track: begin, n_os x (process, sort-reduce-bin), end)."""
+def time_plain(w, host, stride, n_os, workers=1):
+    """Plain FP64 drop-in path: per track begin, n_os x (process, sort-reduce-bin), end.
+
+    Tracks are split into contiguous ranges over `workers` workers (the
+    reference's [Run] Threads); each worker has its own method, histogram,
+    buffers and HIP stream, and the worker histograms are summed at the end
+    (Histogram::ReduceVector)."""
+    import threading
     import supertrack_amd as st
     from supertrack_amd import workloads as W
     x, y, z, e, off, gt = host
     tracks = list(range(0, len(off) - 1, stride))
     geoms = W.geometries(w)
     kind, nb, lo, hi = w["hists"][0]
-    h = st.Histogram(nb, lo, hi, kind)
-    methods = [st.VoxelConstrainedSphereMethod(g, True, True) for g in geoms]
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipStreamCreate.argtypes = [C.POINTER(C.c_void_p)]
+    hip.hipStreamDestroy.argtypes = [C.c_void_p]
+    hip.hipMemcpyAsync.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p]
     # tracks are loaded to the device outside the timed region (ingest is not part of the path)
     dev = []
     cap = 0
@@ -150,29 +160,73 @@ def time_plain(w, host, stride, n_os):
         b, en = int(off[t]), int(off[t + 1])
         dev.append(st.DeviceArray.from_numpy(np.stack([x[b:en], y[b:en], z[b:en], e[b:en]])))
         cap = max(cap, en - b)
-    trk = st.Track(capacity=cap)
-    pairs = st.VolumeEdepPair(cap)
+    n_w = max(1, min(workers, len(tracks)))
+    bounds = [len(tracks) * k // n_w for k in range(n_w + 1)]
+    state = []
+    for k in range(n_w):
+        stream = C.c_void_p()
+        assert hip.hipStreamCreate(C.byref(stream)) == 0
+        h = st.Histogram(nb, lo, hi, kind)
+        h.set_stream(stream)
+        methods = [st.VoxelConstrainedSphereMethod(g, True, True) for g in geoms]
+        for m in methods:
+            m.set_stream(stream)
+        state.append(dict(stream=stream, h=h, methods=methods, trk=st.Track(capacity=cap),
+                          pairs=st.VolumeEdepPair(cap), err=None))
     st.synchronize()
-    deposits = 0
+
+    process = st.lib().st_vcs_process_oversample
+    hist_add = st.lib().st_hist_add
+
+    def run_worker(k):
+        S = state[k]
+        try:
+            trk, pairs, h = S["trk"], S["pairs"], S["h"]
+            for j in range(bounds[k], bounds[k + 1]):
+                t = tracks[j]
+                n = int(off[t + 1] - off[t])
+                base = dev[j].ptr.value
+                for a, dst in enumerate((trk.c.x, trk.c.y, trk.c.z, trk.c.edep)):
+                    assert hip.hipMemcpyAsync(C.c_void_p(dst), C.c_void_p(base + 8 * n * a), 8 * n, 3,
+                                              S["stream"]) == 0
+                trk.c.n = n
+                m = S["methods"][int(gt[t]) if len(geoms) > 1 else 0]
+                pairs.c.capacity = n
+                m.allocate_track_process(trk, W.SEED, t, n_os)
+                h.allocate_track_process(pairs)
+                # the two per-oversample calls go straight to the C ABI (the
+                # wrappers' Python overhead is of the order of a launch)
+                tc, pc = C.byref(trk.c), C.byref(pairs.c)
+                rc = 0
+                for _ in range(n_os):
+                    rc |= process(m.h, tc, pc)
+                    rc |= hist_add(h.h, pc)
+                if rc:
+                    raise RuntimeError(f"plain path call failed: {st.lib().st_last_error()}")
+                m.free_track_process()
+                h.free_track_process()
+        except Exception as ex:   # reported after the join
+            S["err"] = ex
+
     t0 = time.perf_counter()
-    for j, t in enumerate(tracks):
-        n = int(off[t + 1] - off[t])
-        base = dev[j].ptr.value
-        trk.copy_from_device([base, base + 8 * n, base + 16 * n, base + 24 * n], n)
-        m = methods[int(gt[t]) if len(geoms) > 1 else 0]
-        pairs.c.capacity = n
-        m.allocate_track_process(trk, W.SEED, t, n_os)
-        h.allocate_track_process(pairs)
-        for _ in range(n_os):
-            m.process_track(trk, pairs)
-            h.sort_reduce_and_add_to_histogram(pairs)
-        m.free_track_process()
-        h.free_track_process()
-        deposits += n * n_os
+    threads = [threading.Thread(target=run_worker, args=(k,)) for k in range(n_w)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join()
     st.synchronize()
     secs = time.perf_counter() - t0
-    counts = h.counts()
-    pairs.c.capacity = cap
+    for S in state:
+        if S["err"] is not None:
+            raise S["err"]
+    counts = sum(S["h"].counts() for S in state)
+    deposits = sum(int(off[t + 1] - off[t]) for t in tracks) * n_os
+    for S in state:
+        S["pairs"].c.capacity = cap
+        S["h"].free()
+        for m in S["methods"]:
+            m.free()
+        hip.hipStreamDestroy(S["stream"])
     return deposits, secs, len(tracks), counts
 
 
@@ -220,10 +274,10 @@ def plain_main(args, w, n_tracks, n_os):
     total_points = int(host[4][-1])
     deposits = total_points * n_os
     for _ in range(args.warmup):
-        time_plain(w, host, 1, n_os)
+        time_plain(w, host, 1, n_os, args.workers)
     times = []
     for _ in range(args.steps):
-        d, secs, nt, counts = time_plain(w, host, 1, n_os)
+        d, secs, nt, counts = time_plain(w, host, 1, n_os, args.workers)
         times.append(secs)
     secs = float(np.mean(times))
     value = deposits / secs
@@ -241,7 +295,7 @@ def plain_main(args, w, n_tracks, n_os):
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"{args.workload}: {n_tracks} tracks, {total_points} points, {n_os} oversamples, "
                                f"geometries {w['geometries']}, histograms {w['hists'][:1]}",
-                   "deposits_per_step": deposits, "parallelism": "1 GPU"},
+                   "deposits_per_step": deposits, "parallelism": f"1 GPU, {args.workers} workers (one stream each)"},
         "roofline": {"bound": "hbm", "achieved": deposits * 32 / secs / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": deposits * 32 / secs / 1e9 / HBM_PEAK_GBS, "traffic": None,
                      "basis": "32 B per deposit / host-clock step time (per-track launch sequences)"},
@@ -300,13 +354,14 @@ def main():
         lib3f = st.TrackLibrary(*host3[:5], W.geometries(w3), None, st.ST_F32)
         s32, k32, _, _ = time_batch(lib3f, w3, w3["n_oversamples"], 1, 1, False, ev)
         lib3f.free()
-        pd, ps, pn, _ = time_plain(w3, host3, args.plain_stride, w3["n_oversamples"])
+        pd, ps, pn, _ = time_plain(w3, host3, args.plain_stride, w3["n_oversamples"], args.workers)
         plain_rate = pd / ps
         extra = {
             "cfg3_batched_fp64_deposits_per_s": d3 / s64,
             "cfg3_batched_fp32_deposits_per_s": d3 / s32,
             "cfg3_plain_fp64_deposits_per_s": plain_rate,
-            "cfg3_plain_fp64_sample": f"every {args.plain_stride}th track ({pn} tracks) x {w3['n_oversamples']} oversamples",
+            "cfg3_plain_fp64_sample": f"every {args.plain_stride}th track ({pn} tracks) x {w3['n_oversamples']} oversamples, "
+                                      f"{args.workers} workers",
             "cfg3_speedup_batched_fp32_vs_plain_fp64": (d3 / s32) / plain_rate,
         }
         log(f"cfg3: batched fp64 {d3 / s64:.3e}/s, batched fp32 {d3 / s32:.3e}/s, plain fp64 {plain_rate:.3e}/s "

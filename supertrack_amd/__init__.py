@@ -228,6 +228,10 @@ class VoxelConstrainedSphereMethod:
     def free_track_process(self):
         call("st_vcs_end_track", self.h)
 
+    def set_stream(self, stream: C.c_void_p):
+        """Runs this method's work on a given hipStream_t (one per worker)."""
+        call("st_vcs_set_stream", self.h, stream)
+
     def free(self):
         if self.h:
             lib().st_vcs_destroy(self.h)
@@ -266,6 +270,10 @@ class Histogram:
 
     def free_track_process(self):
         call("st_hist_end_track", self.h, self.cpu.ctypes.data_as(_capi.c_double_p))
+
+    def set_stream(self, stream: C.c_void_p):
+        """Runs this histogram's work on a given hipStream_t (one per worker)."""
+        call("st_hist_set_stream", self.h, stream)
 
     def fetch(self, reset: bool = True) -> np.ndarray:
         out = np.zeros(self.nbins, np.uint64)
