@@ -129,6 +129,7 @@ def time_batch(lib, w, n_os, steps, warmup, autotune, ev):
     secs = (time.perf_counter() - t0) / steps
     params = b.params()
     params["exact_path_ms"] = float(np.mean(exact_ms))
+    params.update(b.stats())
     b.free()
     for h in hs:
         h.free()
@@ -329,7 +330,9 @@ def main():
     deposits = total_points * n_os
     secs, kms, params, counts = time_batch(lib, w, n_os, args.steps, args.warmup, args.autotune, ev)
     value = deposits / secs
-    bytes_per_dep = 16 if w["precision"] == st.ST_F32 else 32
+    # algorithmic bytes: every point once per oversample, plus the FP64 side
+    # copy (32 B) of the points whose FP32 decision was rechecked
+    bytes_per_dep = (16 + 32 * params["fp64_rechecks"] / deposits) if w["precision"] == st.ST_F32 else 32
     achieved = deposits * bytes_per_dep / (kms * 1e-3) / 1e9
     log(f"{args.workload} batched: {secs * 1e3:.1f} ms/step, kernel {kms:.1f} ms, {value:.3e} deposits/s, "
         f"params {params}")
@@ -392,10 +395,13 @@ def main():
         "config": {"workload": f"{args.workload}: {n_tracks} tracks, {total_points} points, {n_os} oversamples, "
                                f"geometries {w['geometries']}, histograms {w['hists']}",
                    "deposits_per_step": deposits, "parallelism": "1 GPU", "launch": params,
-                   "fast_tracks": info["fast_tracks"], "exact_tracks": info["exact_tracks"]},
+                   "fast_tracks": info["fast_tracks"], "exact_tracks": info["exact_tracks"],
+                   "scored_fraction": params["points_scored"] / deposits,
+                   "fp64_recheck_fraction_of_scored": params["fp64_rechecks"] / max(params["points_scored"], 1),
+                   "in_sphere_fraction": params["points_in_sphere"] / deposits},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "basis": f"{bytes_per_dep} B per deposit x {deposits} deposits / batch kernel time "
+                     "basis": f"{bytes_per_dep:.6g} B per deposit x {deposits} deposits / batch kernel time "
                               f"({kms:.2f} ms average over {args.steps} launches, HIP events on the batch stream)"},
         "cpu_baseline": cpu,
     }

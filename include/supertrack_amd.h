@@ -202,6 +202,12 @@ int st_batch_times(st_batch *b, float *ms2);
 /* Launch parameters in use: block_threads, table_slots, blocks_per_cu,
  * points_per_thread. */
 int st_batch_params(const st_batch *b, int32_t *params4);
+/* Counts of the last st_batch_run's fused kernel: stats3[0] points scored
+ * (points of the in-voxel block ranges of the segments that pass the x/y
+ * cull), stats3[1] of those whose FP32 decision was within the error bound
+ * and was redone from the FP64 side copy (ST_F32), stats3[2] points inside a
+ * sphere.  Waits for the run.  No counterpart in the reference: measurement. */
+int st_batch_stats(st_batch *b, int64_t *stats3);
 /* Test hook: record (segment, key, sum) of every scored sphere of the next
  * st_batch_run into host-visible arrays of the given capacity (segment =
  * track * n_oversamples + oversample).  capacity 0 disables recording. */

@@ -371,6 +371,12 @@ class BatchRun:
         call("st_batch_times", self.h, t)
         return float(t[0]), float(t[1])
 
+    def stats(self):
+        """Point counts of the last run: scored, FP64 rechecks, inside a sphere."""
+        c = np.zeros(3, np.int64)
+        call("st_batch_stats", self.h, c.ctypes.data_as(_capi.c_int64_p))
+        return {"points_scored": int(c[0]), "fp64_rechecks": int(c[1]), "points_in_sphere": int(c[2])}
+
     def params(self):
         p = np.zeros(4, np.int32)
         call("st_batch_params", self.h, p.ctypes.data_as(_capi.c_int32_p))

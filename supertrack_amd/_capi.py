@@ -109,6 +109,7 @@ PROTOTYPES = {
     "st_batch_sync": (C.c_int, [C.c_void_p]),
     "st_batch_params": (C.c_int, [C.c_void_p, c_int32_p]),
     "st_batch_times": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    "st_batch_stats": (C.c_int, [C.c_void_p, c_int64_p]),
     "st_batch_record": (C.c_int, [C.c_void_p, C.c_int64]),
     "st_batch_record_fetch": (C.c_int, [C.c_void_p, c_int64_p, c_uint64_p, c_double_p, c_int64_p]),
     "st_batch_destroy": (None, [C.c_void_p]),

@@ -83,12 +83,16 @@ struct GeomDev {
     unsigned long long N, N2;
 };
 
-// ST_F32 block record: FP64 origin of the block's FP32 offsets and the bound
-// dl on |approximate - reference| for any shifted, offset coordinate of the
-// block (FP32 rounding of the offsets plus FP64 rounding of every step, for
-// any shift of magnitude <= H').
+// ST_F32 block record (64 B, read through the scalar cache): FP64 origin of
+// the block's FP32 offsets (z with H' added) and the decision thresholds
+// widened by the bound dl on |approximate - reference| for any shifted,
+// offset coordinate of the block (FP32 rounding of the offsets plus FP64
+// rounding of every step, for any shift of magnitude <= H'):
+//   |xs| < h_in on every axis: inside the voxel; |xs| > h_out on any: outside;
+//   |q - rint(q)| >= fm: within dl of a half-way plane;
+//   |dist - r^2| <= mrg: within the error of the squared distance.
 struct BlockF32 {
-    double ox, oy, oz, dl;
+    double ox, oy, ozh, dl, h_in, h_out, fm, mrg;
 };
 
 struct HistDev {
@@ -115,6 +119,7 @@ struct BatchArgs {
     uint64_t seed;
     unsigned long long *ticket;
     int *error;
+    unsigned long long *stats;   // [3]: points scored, exact FP64 rechecks, in-sphere points
     // histograms
     HistDev hist[kMaxHists];
     int n_hists;
@@ -149,8 +154,12 @@ __device__ __forceinline__ BlockF32 load_block(const BlockF32 *p, int64_t idx)
     BlockF32 r;
     r.ox = q[0];
     r.oy = q[1];
-    r.oz = q[2];
+    r.ozh = q[2];
     r.dl = q[3];
+    r.h_in = q[4];
+    r.h_out = q[5];
+    r.fm = q[6];
+    r.mrg = q[7];
     return r;
 }
 
@@ -311,6 +320,8 @@ void batch_kernel(BatchArgs a)
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
+    // per-thread statistics, summed at exit
+    unsigned int n_scored = 0, n_flagged = 0, n_in = 0;
     for (int s = tid; s < SLOTS; s += kBT) {
         tkey[s] = kEmpty;
         tacc0[s] = 0;
@@ -374,17 +385,16 @@ void batch_kernel(BatchArgs a)
                 const int idx = tid + r * kBT;
                 if (idx < n) {
                     const int sl = min((int)tlist[idx], SLOTS - 1);
-                    const unsigned long long k = tkey[sl];
-                    const unsigned long long v0 = tacc0[sl];
-                    const unsigned long long v1 = (kWords == 2) ? tacc1[sl] : 0ull;
+                    // read and clear in one exchange each (the slot is this
+                    // thread's alone until the barrier)
+                    const unsigned long long k = atomicExch(&tkey[sl], kEmpty);
+                    const unsigned long long v0 = atomicExch(&tacc0[sl], 0ull);
+                    const unsigned long long v1 = (kWords == 2) ? atomicExch(&tacc1[sl], 0ull) : 0ull;
                     em[u] = k < bound;
                     if (!em[u]) kk[r] = k;
                     k0[r] = v0;
                     k1[r] = v1;
                     vv[u] = Acc<MODE>::value(v0, v1);
-                    tkey[sl] = kEmpty;
-                    tacc0[sl] = 0;
-                    if (kWords == 2) tacc1[sl] = 0;
                     if (em[u] && a.rec_cap > 0) {
                         const unsigned long long p = atomicAdd(a.rec_count, 1ull);
                         if ((int64_t)p < a.rec_cap) {
@@ -526,13 +536,10 @@ void batch_kernel(BatchArgs a)
                 } else {
                     // block-uniform terms (scalar loads)
                     const BlockF32 bk = load_block(a.blocks, td.block0 + b + j);
-                    const double dl = bk.dl;
                     const double cx = bk.ox + sh.x;
                     const double cy = bk.oy + sh.y;
-                    const double cz = (bk.oz + g.H) + sh.z;
-                    const double h_in = g.H - dl, h_out = g.H + dl;
-                    const double fm = 0.5 - (dl * g.invD + 1e-12);
-                    const double mrg = 3.0 * dl * (g.D + 2.0 * dl) + 1e-14 * g.r2;
+                    const double cz = bk.ozh + sh.z;
+                    const double h_in = bk.h_in, h_out = bk.h_out, fm = bk.fm, mrg = bk.mrg;
                     // per point: approximate shifted coordinates, |error| <= dl
                     const double xs = cx + (double)cur[j].x;
                     const double ys = cy + (double)cur[j].y;
@@ -544,8 +551,9 @@ void batch_kernel(BatchArgs a)
                     const double qx = ax * g.invD, qy = ay * g.invD, qz = az * g.invD;
                     const double tx = qx + kRoundMagic, ty = qy + kRoundMagic, tz = qz + kRoundMagic;
                     const double kx = tx - kRoundMagic, ky = ty - kRoundMagic, kz = tz - kRoundMagic;
-                    const double dx = kx * g.D - ax, dy = ky * g.D - ay, dz = kz * g.D - az;
-                    const double dist = dx * dx + dy * dy + dz * dz;
+                    // fused products: one rounding each, inside the bound mrg
+                    const double dx = fma(kx, g.D, -ax), dy = fma(ky, g.D, -ay), dz = fma(kz, g.D, -az);
+                    const double dist = fma(dx, dx, fma(dy, dy, dz * dz));
                     const bool near = fabs(qx - kx) >= fm || fabs(qy - ky) >= fm || fabs(qz - kz) >= fm ||
                                       fabs(dist - g.r2) <= mrg;
                     // any decision within the error bound is redone exactly from FP64
@@ -554,9 +562,12 @@ void batch_kernel(BatchArgs a)
                     // low words of t hold the (non-negative) indices
                     key = lattice_key((unsigned)__double2loint(tx), (unsigned)__double2loint(ty),
                                       (unsigned)__double2loint(tz), (unsigned)g.N);
+                    n_flagged += amb;
                     if (amb) in = exact_member(a.x64[i], a.y64[i], a.z64[i] + g.H, g, sh.x, sh.y, sh.z, key);
                     v0 = rint_to_i64((double)cur[j].w * g.yscale);
                 }
+                n_scored += valid;
+                n_in += in;
                 if (in) {
                     unsigned int h = slot_hash<SLOTS>(key);
                     unsigned long long prev = 0;
@@ -607,6 +618,20 @@ void batch_kernel(BatchArgs a)
             binned = 0;
         }
     }
+    {
+        unsigned long long c0 = n_scored, c1 = n_flagged, c2 = n_in;
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) {
+            c0 += __shfl_xor(c0, d, 64);
+            c1 += __shfl_xor(c1, d, 64);
+            c2 += __shfl_xor(c2, d, 64);
+        }
+        if (lane == 0) {
+            atomicAdd(&a.stats[0], c0);
+            atomicAdd(&a.stats[1], c1);
+            atomicAdd(&a.stats[2], c2);
+        }
+    }
     __syncthreads();
     if (a.hist_lds_words == 0) return;
 #pragma unroll
@@ -620,12 +645,14 @@ void batch_kernel(BatchArgs a)
     }
 }
 
-__global__ void batch_reset_kernel(unsigned long long *ticket, int *error, unsigned long long *rec_count)
+__global__ void batch_reset_kernel(unsigned long long *ticket, int *error, unsigned long long *rec_count,
+                                   unsigned long long *stats)
 {
     if (threadIdx.x == 0) {
         *ticket = 0;
         *error = 0;
         if (rec_count) *rec_count = 0;
+        stats[0] = stats[1] = stats[2] = 0;
     }
 }
 
@@ -738,6 +765,7 @@ struct st_batch {
     unsigned long long *ticket = nullptr;
     int *error = nullptr;
     unsigned long long *rec_count = nullptr;
+    unsigned long long *stats = nullptr;   // [3] per run, see BatchArgs
     int64_t rec_cap = 0;
     int64_t *rec_seg = nullptr;
     unsigned long long *rec_key = nullptr;
@@ -789,6 +817,7 @@ BatchArgs make_args(st_batch *b)
     a.hist_lds_words = b->lds_hist ? off : 0;
     a.rec_cap = b->rec_cap;
     a.rec_count = b->rec_count;
+    a.stats = b->stats;
     a.rec_seg = b->rec_seg;
     a.rec_key = b->rec_key;
     a.rec_sum = b->rec_sum;
@@ -798,7 +827,8 @@ BatchArgs make_args(st_batch *b)
 int launch_fast(st_batch *b, int slots, int grid)
 {
     BatchArgs a = make_args(b);
-    batch_reset_kernel<<<1, 64, 0, b->stream>>>(b->ticket, b->error, b->rec_cap > 0 ? b->rec_count : nullptr);
+    batch_reset_kernel<<<1, 64, 0, b->stream>>>(b->ticket, b->error, b->rec_cap > 0 ? b->rec_count : nullptr,
+                                                 b->stats);
     if (a.n_seg == 0) return ST_OK;
     const size_t lds = lds_bytes(b->lib->precision, slots, b->lds_hist ? b->hist_words : 0);
     const void *k = pick_kernel(b->lib->precision, slots, b->ppt);
@@ -1009,10 +1039,15 @@ int st_library_create(const double *x, const double *y, const double *z, const d
                 BlockF32 &bk = blocks[td.block0 + b];
                 bk.ox = org[0];
                 bk.oy = org[1];
-                bk.oz = org[2];
-                bk.dl = err + kTwoM44 * (std::fabs(org[0]) + std::fabs(org[1]) + std::fabs(org[2]) +
-                                         7.0 * g.half_length + std::fabs(g.gso) + g.diameter + 16.0);
-                if (!std::isfinite(bk.dl)) bk.dl = HUGE_VAL;
+                bk.ozh = org[2] + g.half_length;
+                double dl = err + kTwoM44 * (std::fabs(org[0]) + std::fabs(org[1]) + std::fabs(org[2]) +
+                                             7.0 * g.half_length + std::fabs(g.gso) + g.diameter + 16.0);
+                if (!std::isfinite(dl)) dl = HUGE_VAL;
+                bk.dl = dl;
+                bk.h_in = g.half_length - dl;
+                bk.h_out = g.half_length + dl;
+                bk.fm = 0.5 - (dl / g.diameter + 1e-12);
+                bk.mrg = 3.0 * dl * (g.diameter + 2.0 * dl) + 1e-14 * (g.radius * g.radius);
             }
         }
     });
@@ -1110,6 +1145,8 @@ int st_batch_create(st_library *lib, const st_batch_opts *opts, st_hist *const *
     hipError_t e = hipMalloc((void **)&b->ticket, sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMalloc((void **)&b->error, sizeof(int));
     if (e == hipSuccess) e = hipMalloc((void **)&b->rec_count, sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc((void **)&b->stats, 3 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMemset(b->stats, 0, 3 * sizeof(unsigned long long));
     if (e == hipSuccess && !lib->fast_order.empty())
         e = hipMalloc((void **)&b->order_dev, lib->fast_order.size() * sizeof(int32_t));
     if (e == hipSuccess && !lib->fast_order.empty())
@@ -1251,6 +1288,16 @@ int st_batch_sync(st_batch *b)
     return ST_OK;
 }
 
+int st_batch_stats(st_batch *b, int64_t *stats3)
+{
+    if (!b || !stats3) return set_error(ST_EINVAL, "st_batch_stats: null argument");
+    ST_HIP_TRY(hipStreamSynchronize(b->stream));
+    unsigned long long c[3];
+    ST_HIP_TRY(hipMemcpy(c, b->stats, sizeof c, hipMemcpyDeviceToHost));
+    for (int i = 0; i < 3; ++i) stats3[i] = (int64_t)c[i];
+    return ST_OK;
+}
+
 int st_batch_params(const st_batch *b, int32_t *p)
 {
     if (!b || !p) return set_error(ST_EINVAL, "st_batch_params: null argument");
@@ -1305,7 +1352,7 @@ void st_batch_destroy(st_batch *b)
         if (e) (void)hipEventDestroy(e);
     st_track_free(&b->scratch);
     st_pairs_free(&b->pairs);
-    void *ptrs[] = {b->order_dev, b->ticket, b->error, b->rec_count, b->rec_seg, b->rec_key, b->rec_sum};
+    void *ptrs[] = {b->order_dev, b->ticket, b->error, b->rec_count, b->stats, b->rec_seg, b->rec_key, b->rec_sum};
     for (void *p : ptrs) (void)hipFree(p);
     delete b;
 }

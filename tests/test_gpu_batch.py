@@ -39,6 +39,7 @@ def gpu_run(x, y, z, e, off, geoms_hd, gt, n_os, seed, shift_xy, shift_z, precis
     rec = b.record_fetch(record) if record else None
     counts = [h.fetch() for h in hs]
     info = lib.info()
+    info.update(b.stats())
     edges = [h.edges.copy() for h in hs]
     b.free()
     for h in hs:
@@ -259,3 +260,15 @@ def test_batch_points_on_decision_boundaries(precision, diameter):
     n, _ = compare_records(rec, oracle, 1e-6 if precision == st.ST_F32 else 1e-12)
     assert n > 1000
     check_bins(counts, edges, oracle, 1e-6 if precision == st.ST_F32 else 1e-12)
+    # run statistics: in-sphere points equal the oracle's pair count; the
+    # points placed on boundaries are rechecked from FP64 (ST_F32 only)
+    n_pairs = sum(len(pk) for t in range(n_tracks)
+                  for pk, _, _, _ in po.track_segments(x[off[t]:off[t + 1]], y[off[t]:off[t + 1]],
+                                                       z[off[t]:off[t + 1]], e[off[t]:off[t + 1]], g, W.SEED, t,
+                                                       n_os, True, True))
+    assert info["points_in_sphere"] == n_pairs
+    assert n_pairs <= info["points_scored"] <= n_tracks * n_os * n_os * per_os
+    if precision == st.ST_F32:
+        assert info["fp64_rechecks"] > 0.2 * n_tracks * n_os * per_os
+    else:
+        assert info["fp64_rechecks"] == 0
