@@ -183,8 +183,8 @@ typedef struct {
                                      0 = 1536                                   */
     int32_t blocks_per_cu;        /* persistent workgroups per CU; 0 = 5 (ST_F32),
                                      3 (ST_F64), capped by occupancy           */
-    int32_t points_per_thread;    /* 1 or 2 per iteration; 0 = 1 (ST_F32),
-                                     2 (ST_F64)                                */
+    int32_t points_per_thread;    /* 1 or 2 per iteration; 0 = 1 (ST_F32 at five
+                                     workgroups per CU, else 2), 2 (ST_F64)    */
 } st_batch_opts;
 
 typedef struct st_batch st_batch;

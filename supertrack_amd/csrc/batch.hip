@@ -1173,6 +1173,12 @@ int st_batch_create(st_library *lib, const st_batch_opts *opts, st_hist *const *
     if (ppt != 1 && ppt != 2) return fail(set_error(ST_EINVAL, "st_batch_create: points_per_thread must be 1 or 2"));
     rc = configure(b, slots, bpc, ppt);
     if (rc) return fail(rc);
+    // FP32 with fewer than five workgroups per CU (large privatised
+    // histograms): two points per thread hide more latency
+    if (f32 && !opts->points_per_thread && b->blocks_per_cu < 5) {
+        rc = configure(b, slots, bpc, 2);
+        if (rc) return fail(rc);
+    }
     if (opts->autotune && !lib->fast_order.empty() && opts->n_oversamples > 0) {
         // time each candidate on one full pass; keep the fastest
         const int cand_slots[] = {1536, 2048, 4096};
