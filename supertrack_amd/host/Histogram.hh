@@ -45,6 +45,8 @@ public:
     void AllocateTrackProcess(VolumeEdepPair targetEdeps);
     void FreeTrackProcess();
     void SortReduceAndAddToHistogram(VolumeEdepPair targetEdeps);
+    // device work of this histogram on a given hipStream_t (one per worker)
+    void SetStream(void *stream);
     void Print();
 
     Hist1D GetCPUHistogram() { return _CPU_histogram; }

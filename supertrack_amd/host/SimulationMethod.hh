@@ -23,6 +23,9 @@ public:
     virtual void ProcessTrack(Track track, VolumeEdepPair &edepsInTarget) = 0;
     virtual void FreeTrackProcess() = 0;
     virtual void Free() = 0;
+    // Runs the method's device work on a given hipStream_t, so that workers
+    // can run concurrently (the reference's workers are separate processes).
+    virtual void SetStream(void *stream) { (void)stream; }
 
 protected:
     INIReader _macroReader;

@@ -1,8 +1,8 @@
 // SuperTrackManager.hh -- run driver: tasks from the macro's folder, the
 // per-track loop of SuperTrackManager.cu:58-83 and the output file
 // (SuperTrackManager.hh:14-37).  One process, one GPU: the reference's forked
-// workers become sequential allocations whose histograms are summed
-// (Histogram::ReduceVector).  With [VoxelConstrainedSphere] Batched = true the
+// workers become host threads, each with its own HIP stream, whose
+// histograms are summed (Histogram::ReduceVector).  With [VoxelConstrainedSphere] Batched = true the
 // whole task list is scored by one batched pass instead.
 #pragma once
 

@@ -12,6 +12,9 @@ public:
     // Reads the task's entries from its track file and copies them to the GPU.
     void AllocateAndLoadTrack(const ThreadTask &task);
     void AllocateEmptyTrack(int nVals);
+    // Reads the task's entries into this track's buffers (capacity must
+    // hold them) by copies on the given hipStream_t; returns when done.
+    void LoadTrack(const ThreadTask &task, void *stream);
 
     double *x = nullptr;
     double *y = nullptr;

@@ -20,6 +20,7 @@ public:
     void ProcessTrack(Track track, VolumeEdepPair &edepsInTarget) override;
     void FreeTrackProcess() override;
     void Free() override;
+    void SetStream(void *stream) override;
 
     static SimulationMethod *Construct(const INIReader &macroReader);
 

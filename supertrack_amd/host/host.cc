@@ -9,6 +9,9 @@
 #include <filesystem>
 #include <iostream>
 #include <sstream>
+#include <thread>
+
+#include <hip/hip_runtime_api.h>
 
 #include "Histogram.hh"
 #include "SimulationMethod.hh"
@@ -221,6 +224,30 @@ void Track::AllocateAndLoadTrack(const ThreadTask &task)
     edep = c.edep;
 }
 
+void Track::LoadTrack(const ThreadTask &task, void *stream)
+{
+    TrackFile::Header h;
+    if (!TrackFile::ReadHeader(task.GetFilename(), h)) fatal("Cannot read track file " + task.GetFilename());
+    const int64_t n = task.GetExitPoint() - task.GetEntryPoint();
+    if (n > c.capacity) fatal("Track::LoadTrack: track larger than the allocated capacity");
+    std::vector<double> hx(n), hy(n), hz(n), he(n);
+    if (!TrackFile::ReadRange(task.GetFilename(), h, task.GetEntryPoint(), task.GetExitPoint(), hx.data(), hy.data(),
+                              hz.data(), he.data()))
+        fatal("Cannot read track entries from " + task.GetFilename());
+    const hipStream_t s = (hipStream_t)stream;
+    const size_t bytes = (size_t)n * sizeof(double);
+    bool ok = true;
+    if (n > 0) {
+        ok = ok && hipMemcpyAsync(c.x, hx.data(), bytes, hipMemcpyHostToDevice, s) == hipSuccess;
+        ok = ok && hipMemcpyAsync(c.y, hy.data(), bytes, hipMemcpyHostToDevice, s) == hipSuccess;
+        ok = ok && hipMemcpyAsync(c.z, hz.data(), bytes, hipMemcpyHostToDevice, s) == hipSuccess;
+        ok = ok && hipMemcpyAsync(c.edep, he.data(), bytes, hipMemcpyHostToDevice, s) == hipSuccess;
+    }
+    ok = ok && hipStreamSynchronize(s) == hipSuccess;
+    if (!ok) fatal("Track::LoadTrack: copy to the device failed");
+    c.n = n;
+}
+
 void Track::Free()
 {
     st_track_free(&c);
@@ -306,6 +333,11 @@ void VoxelConstrainedSphereMethod::FreeTrackProcess()
 
 void VoxelConstrainedSphereMethod::Free() {}
 
+void VoxelConstrainedSphereMethod::SetStream(void *stream)
+{
+    check(st_vcs_set_stream(_vcs, stream), "VoxelConstrainedSphereMethod::SetStream");
+}
+
 // -------------------------------------------------------------- Hist1D --
 Hist1D::Hist1D(const std::string &name, const std::vector<double> &edges)
     : _name(name), _edges(edges), _contents(edges.size() + 1, 0.0)
@@ -385,6 +417,11 @@ void Histogram::SortReduceAndAddToHistogram(VolumeEdepPair targetEdeps)
     check(st_hist_add(_hist, &targetEdeps.c), "Histogram::SortReduceAndAddToHistogram");
 }
 
+void Histogram::SetStream(void *stream)
+{
+    check(st_hist_set_stream(_hist, stream), "Histogram::SetStream");
+}
+
 void Histogram::FreeTrackProcess()
 {
     std::vector<double> add(_nbins + 2, 0.0);
@@ -455,30 +492,50 @@ void SuperTrackManager::Run()
 
 void SuperTrackManager::RunPerTrack()
 {
-    std::vector<Hist1D> outputs;
-    for (const ThreadAllocation &alloc : _threadAllocations) {
+    // One host thread per worker (the reference forks one process per
+    // worker), each with its own stream, method, histogram and buffers sized
+    // for its largest task; the worker histograms are summed at the end.
+    const size_t nw = _threadAllocations.size();
+    std::vector<Hist1D> outputs(nw);
+    auto work = [&](size_t k) {
+        const ThreadAllocation &alloc = _threadAllocations[k];
+        hipStream_t stream = nullptr;
+        if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) fatal("hipStreamCreate failed");
         Histogram histogram(*_inputFileReader);
+        histogram.SetStream(stream);
         SimulationMethod *method = SimulationMethodFactory::GetInstance().Construct(*_inputFileReader);
+        method->SetStream(stream);
+        int64_t cap = 1;
+        for (const ThreadTask &task : alloc.GetTasks())
+            cap = std::max<int64_t>(cap, task.GetExitPoint() - task.GetEntryPoint());
+        Track track;
+        track.AllocateEmptyTrack((int)cap);
+        VolumeEdepPair edepsInTarget;
+        edepsInTarget.Allocate((uint64_t)cap);
         for (ThreadTask task : alloc.GetTasks()) {
-            Track track;
-            track.AllocateAndLoadTrack(task);
-            VolumeEdepPair edepsInTarget;
-            edepsInTarget.Allocate(task.GetExitPoint() - task.GetEntryPoint());
+            const int64_t n = task.GetExitPoint() - task.GetEntryPoint();
+            track.LoadTrack(task, stream);
+            edepsInTarget.c.capacity = std::max<int64_t>(n, 1);
             method->AllocateTrackProcess(track, task);
             histogram.AllocateTrackProcess(edepsInTarget);
             for (int os = 0; os < task.GetNOversamples(); ++os) {
                 method->ProcessTrack(track, edepsInTarget);
                 histogram.SortReduceAndAddToHistogram(edepsInTarget);
             }
-            check(st_device_synchronize(), "cudaDeviceSynchronize equivalent");
+            if (hipStreamSynchronize(stream) != hipSuccess) fatal("hipStreamSynchronize failed");
             method->FreeTrackProcess();
             histogram.FreeTrackProcess();
-            track.Free();
-            edepsInTarget.Free();
         }
+        edepsInTarget.c.capacity = cap;
         delete method;
-        outputs.push_back(histogram.GetCPUHistogram());
-    }
+        outputs[k] = histogram.GetCPUHistogram();
+        track.Free();
+        edepsInTarget.Free();
+        (void)hipStreamDestroy(stream);
+    };
+    std::vector<std::thread> threads;
+    for (size_t k = 0; k < nw; ++k) threads.emplace_back(work, k);
+    for (std::thread &t : threads) t.join();
     if (outputs.empty()) {
         Histogram empty(*_inputFileReader);
         _output = empty.GetCPUHistogram();
