@@ -22,7 +22,7 @@
 // chunks of 256 points.  In-sphere points are accumulated per sphere key in an
 // LDS open-addressing table as fixed-point integers (exact and independent of
 // order, so results are bit-reproducible and do not depend on launch
-// parameters).  When the table fills past half, every entry whose key lies
+// parameters).  When the table fills past a quarter, every entry whose key lies
 // below the first layer still to come is complete: those are converted to
 // lineal energy and binned into LDS-privatised histograms; the rest are kept.
 // Tracks whose densest 4-layer window could hold more spheres than the table
@@ -299,11 +299,17 @@ void batch_kernel(BatchArgs a)
 {
     static_assert(SLOTS == 1536 || SLOTS == 2048 || SLOTS == 4096, "table sizes");
     // occupancy never exceeds max(half table, survivors) + one iteration's points
-    static_assert((SLOTS / 2 > kMaxWindow ? SLOTS / 2 : kMaxWindow) + kPPT * kBlk <= SLOTS * 9 / 10,
+    // A flush starts once the table is a quarter full: a low load factor
+    // keeps the linear-probe chains (dependent LDS round trips, paid by the
+    // whole wave for its longest lane) short; a quarter was fastest on MI355X
+    // (an eighth flushes too often, a half probes too long).
+    constexpr int kFlushAt = SLOTS / 4;
+    // occupancy never exceeds max(flush threshold, survivors) + one iteration's points
+    static_assert((kFlushAt > kMaxWindow ? kFlushAt : kMaxWindow) + kPPT * kBlk <= SLOTS * 9 / 10,
                   "the table must hold the survivors of a flush plus an iteration");
     // entries at a flush: those kept by the previous flush (at most kMaxWindow)
-    // or below SLOTS / 2, plus the points of the last iteration
-    constexpr int kMaxEntries = (SLOTS / 2 - 1 > kMaxWindow ? SLOTS / 2 - 1 : (int)kMaxWindow) + kPPT * kBlk;
+    // or below the threshold, plus the points of the last iteration
+    constexpr int kMaxEntries = (kFlushAt - 1 > kMaxWindow ? kFlushAt - 1 : (int)kMaxWindow) + kPPT * kBlk;
     constexpr int kLP = (kMaxEntries + kBT - 1) / kBT;   // list entries per thread
     constexpr int kWords = Acc<MODE>::words;
     extern __shared__ unsigned long long lds[];
@@ -588,7 +594,7 @@ void batch_kernel(BatchArgs a)
             append(fresh, std::integral_constant<int, kPPT>());
             __syncthreads();
             const int64_t next = c + (int64_t)kPPT * kBlk;
-            if (s_occ >= SLOTS / 2 && next < p_end) {
+            if (s_occ >= kFlushAt && next < p_end) {
                 // every later point lies in a layer >= the next point's exact
                 // layer index (z order), and a key's layer (key / N^2) is never
                 // below its point's layer: entries with smaller keys are complete
