@@ -204,3 +204,59 @@ def test_reference_a7_pi_estimate():
     bins, _, _ = po.run(x, y, z, e, off, [po.geometry(1.5e6, 1e3)], None, 10000, 20240601, True, False,
                         [po.edges("lin", 1, 1e4, 10000)])
     np.testing.assert_array_equal(h.counts(), bins[0].astype(np.float64))
+
+
+def test_plain_concurrent_workers_on_streams_match_oracle():
+    """Workers on their own HIP streams in separate host threads (the
+    reference's forked workers): the summed histogram equals the oracle's."""
+    import ctypes as C
+    import threading
+    w = W.workload("cfg2")
+    x, y, z, e, off, _ = W.make_library(w, n_tracks=18)
+    kind, nb, lo, hi = w["hists"][0]
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipStreamCreate.argtypes = [C.POINTER(C.c_void_p)]
+    hip.hipStreamDestroy.argtypes = [C.c_void_p]
+    n_w = 3
+    hs, errs = [], []
+
+    def worker(k, stream):
+        try:
+            h = st.Histogram(nb, lo, hi, kind)
+            h.set_stream(stream)
+            m = st.VoxelConstrainedSphereMethod(st.SphericalGeometry(2.5e3, 10.0), True, True)
+            m.set_stream(stream)
+            for t in range(k, 18, n_w):
+                b, en = off[t], off[t + 1]
+                trk = st.Track(x[b:en], y[b:en], z[b:en], e[b:en])
+                pairs = st.VolumeEdepPair(en - b)
+                m.allocate_track_process(trk, W.SEED, t, 20)
+                h.allocate_track_process(pairs)
+                for _ in range(20):
+                    m.process_track(trk, pairs)
+                    h.sort_reduce_and_add_to_histogram(pairs)
+                m.free_track_process()
+                h.free_track_process()
+            hs.append(h.counts())
+            m.free()
+            h.free()
+        except Exception as ex:   # reported by the main thread
+            errs.append(ex)
+
+    streams = []
+    for _ in range(n_w):
+        s = C.c_void_p()
+        assert hip.hipStreamCreate(C.byref(s)) == 0
+        streams.append(s)
+    threads = [threading.Thread(target=worker, args=(k, streams[k])) for k in range(n_w)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join()
+    st.synchronize()
+    for s in streams:
+        hip.hipStreamDestroy(s)
+    assert not errs, errs
+    bins, _, _ = po.run(x, y, z, e, off, [po.geometry(2.5e3, 10.0)], None, 20, W.SEED, True, True,
+                        [po.edges(kind, lo, hi, nb)])
+    np.testing.assert_array_equal(sum(hs), bins[0].astype(np.float64))
