@@ -80,19 +80,25 @@ struct TrackDesc {
 struct GeomDev {
     double H, L, D, invD, r2, gso, denom, denom_inv;
     double yscale;      // denom_inv * 2^32 (ST_F32 fixed point)
+    double qc;          // gso / D: lattice units of the voxel centre offset
+    double r2q;         // r^2 / D^2
     unsigned long long N, N2;
 };
 
-// ST_F32 block record (64 B, read through the scalar cache): FP64 origin of
-// the block's FP32 offsets (z with H' added) and the decision thresholds
-// widened by the bound dl on |approximate - reference| for any shifted,
-// offset coordinate of the block (FP32 rounding of the offsets plus FP64
-// rounding of every step, for any shift of magnitude <= H'):
-//   |xs| < h_in on every axis: inside the voxel; |xs| > h_out on any: outside;
-//   |q - rint(q)| >= fm: within dl of a half-way plane;
-//   |dist - r^2| <= mrg: within the error of the squared distance.
+// ST_F32 block record (64 B, read through the scalar cache).  The fast path
+// works in lattice units: q = (coordinate - gso) / D, so rint(q) is the
+// sphere index and q - rint(q) the offset from the sphere centre.
+//   oq*     (origin - gso) / D of the block's FP32 offsets (z with H' added);
+//   dl      bound on |approximate - reference| of any shifted, offset
+//           coordinate of the block in nm (FP32 rounding of the offsets plus,
+//           with a wide margin, FP64 rounding of every step, any |shift| <= H');
+//   hq_in   |q + gso / D| < hq_in on every axis: inside the voxel;
+//   hq_out  |q + gso / D| > hq_out on any axis: outside;
+//   fm      |q - rint(q)| >= fm: within dl of a half-way plane;
+//   mrgq    |sum (rint(q) - q)^2 - r^2 / D^2| <= mrgq: within the error of
+//           the squared distance.
 struct BlockF32 {
-    double ox, oy, ozh, dl, h_in, h_out, fm, mrg;
+    double oqx, oqy, oqz, dl, hq_in, hq_out, fm, mrgq;
 };
 
 struct HistDev {
@@ -152,14 +158,14 @@ __device__ __forceinline__ BlockF32 load_block(const BlockF32 *p, int64_t idx)
     typedef const __attribute__((address_space(4))) double *ConstPtr;
     const ConstPtr q = (ConstPtr)(const void *)(p + idx);
     BlockF32 r;
-    r.ox = q[0];
-    r.oy = q[1];
-    r.ozh = q[2];
+    r.oqx = q[0];
+    r.oqy = q[1];
+    r.oqz = q[2];
     r.dl = q[3];
-    r.h_in = q[4];
-    r.h_out = q[5];
+    r.hq_in = q[4];
+    r.hq_out = q[5];
     r.fm = q[6];
-    r.mrg = q[7];
+    r.mrgq = q[7];
     return r;
 }
 
@@ -477,6 +483,8 @@ void batch_kernel(BatchArgs a)
         const Shift sh = shift_for(a.seed, a.track_index_base + (uint32_t)t, (uint32_t)os, a.shift_xy,
                                    a.shift_z, g.L, g.H);
         const int64_t seg_id = (int64_t)t * a.n_os + os;
+        // the shift in lattice units (ST_F32 fast path)
+        const double sqx = sh.x * g.invD, sqy = sh.y * g.invD, sqz = sh.z * g.invD;
         // x/y bounds cull: fl(x + s) is monotone in x
         if (!((td.xmax + sh.x) > -g.H && (td.xmin + sh.x) < g.H && (td.ymax + sh.y) > -g.H &&
               (td.ymin + sh.y) < g.H))
@@ -542,29 +550,23 @@ void batch_kernel(BatchArgs a)
                 } else {
                     // block-uniform terms (scalar loads)
                     const BlockF32 bk = load_block(a.blocks, td.block0 + b + j);
-                    const double cx = bk.ox + sh.x;
-                    const double cy = bk.oy + sh.y;
-                    const double cz = bk.ozh + sh.z;
-                    const double h_in = bk.h_in, h_out = bk.h_out, fm = bk.fm, mrg = bk.mrg;
-                    // per point: approximate shifted coordinates, |error| <= dl
-                    const double xs = cx + (double)cur[j].x;
-                    const double ys = cy + (double)cur[j].y;
-                    const double zs = cz + (double)cur[j].z;
-                    const double axs = fabs(xs), ays = fabs(ys), azs = fabs(zs);
-                    const bool box_in = axs < h_in && ays < h_in && azs < h_in;
-                    const bool box_out = axs > h_out || ays > h_out || azs > h_out;
-                    const double ax = xs - g.gso, ay = ys - g.gso, az = zs - g.gso;
-                    const double qx = ax * g.invD, qy = ay * g.invD, qz = az * g.invD;
+                    const double cqx = bk.oqx + sqx, cqy = bk.oqy + sqy, cqz = bk.oqz + sqz;
+                    // per point, in lattice units: |q - reference a / D| <= dl / D
+                    const double qx = fma((double)cur[j].x, g.invD, cqx);
+                    const double qy = fma((double)cur[j].y, g.invD, cqy);
+                    const double qz = fma((double)cur[j].z, g.invD, cqz);
+                    const double px = fabs(qx + g.qc), py = fabs(qy + g.qc), pz = fabs(qz + g.qc);
+                    const bool box_in = px < bk.hq_in && py < bk.hq_in && pz < bk.hq_in;
+                    const bool box_out = px > bk.hq_out || py > bk.hq_out || pz > bk.hq_out;
                     const double tx = qx + kRoundMagic, ty = qy + kRoundMagic, tz = qz + kRoundMagic;
-                    const double kx = tx - kRoundMagic, ky = ty - kRoundMagic, kz = tz - kRoundMagic;
-                    // fused products: one rounding each, inside the bound mrg
-                    const double dx = fma(kx, g.D, -ax), dy = fma(ky, g.D, -ay), dz = fma(kz, g.D, -az);
-                    const double dist = fma(dx, dx, fma(dy, dy, dz * dz));
-                    const bool near = fabs(qx - kx) >= fm || fabs(qy - ky) >= fm || fabs(qz - kz) >= fm ||
-                                      fabs(dist - g.r2) <= mrg;
+                    const double ux = (tx - kRoundMagic) - qx, uy = (ty - kRoundMagic) - qy,
+                                 uz = (tz - kRoundMagic) - qz;
+                    const double distq = fma(ux, ux, fma(uy, uy, uz * uz));
+                    const bool near = fabs(ux) >= bk.fm || fabs(uy) >= bk.fm || fabs(uz) >= bk.fm ||
+                                      fabs(distq - g.r2q) <= bk.mrgq;
                     // any decision within the error bound is redone exactly from FP64
                     const bool amb = valid && !box_out && (!box_in || near);
-                    in = valid && box_in && !near && dist < g.r2;
+                    in = valid && box_in && !near && distq < g.r2q;
                     // low words of t hold the (non-negative) indices
                     key = lattice_key((unsigned)__double2loint(tx), (unsigned)__double2loint(ty),
                                       (unsigned)__double2loint(tz), (unsigned)g.N);
@@ -1043,17 +1045,20 @@ int st_library_create(const double *x, const double *y, const double *z, const d
                 // the FP64 rounding of each add and product on the way to the
                 // sphere test (|shift| <= H' on each axis)
                 BlockF32 &bk = blocks[td.block0 + b];
-                bk.ox = org[0];
-                bk.oy = org[1];
-                bk.ozh = org[2] + g.half_length;
+                const double invD = 1.0 / g.diameter;
+                bk.oqx = (org[0] - g.gso) * invD;
+                bk.oqy = (org[1] - g.gso) * invD;
+                bk.oqz = ((org[2] + g.half_length) - g.gso) * invD;
                 double dl = err + kTwoM44 * (std::fabs(org[0]) + std::fabs(org[1]) + std::fabs(org[2]) +
                                              7.0 * g.half_length + std::fabs(g.gso) + g.diameter + 16.0);
                 if (!std::isfinite(dl)) dl = HUGE_VAL;
+                const double dlq = dl * invD;
+                const double r2q = (g.radius * g.radius) / (g.diameter * g.diameter);
                 bk.dl = dl;
-                bk.h_in = g.half_length - dl;
-                bk.h_out = g.half_length + dl;
-                bk.fm = 0.5 - (dl / g.diameter + 1e-12);
-                bk.mrg = 3.0 * dl * (g.diameter + 2.0 * dl) + 1e-14 * (g.radius * g.radius);
+                bk.hq_in = (g.half_length - dl) * invD;
+                bk.hq_out = (g.half_length + dl) * invD;
+                bk.fm = 0.5 - (dlq + 1e-12);
+                bk.mrgq = 3.0 * dlq * (1.0 + 2.0 * dlq) + 1e-14 * r2q;
             }
         }
     });
@@ -1077,6 +1082,8 @@ int st_library_create(const double *x, const double *y, const double *z, const d
         gd[i].denom = (2. / 3.) * g.diameter;
         gd[i].denom_inv = 1.0 / gd[i].denom;
         gd[i].yscale = gd[i].denom_inv * kTwo32;
+        gd[i].qc = g.gso / g.diameter;
+        gd[i].r2q = (g.radius * g.radius) / (g.diameter * g.diameter);
         gd[i].N = (unsigned long long)g.n_linear;
         gd[i].N2 = gd[i].N * gd[i].N;
     }
