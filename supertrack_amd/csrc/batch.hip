@@ -311,11 +311,14 @@ void batch_kernel(BatchArgs a)
     // (an eighth flushes too often, a half probes too long).
     constexpr int kFlushAt = SLOTS / 4;
     // occupancy never exceeds max(flush threshold, survivors) + one iteration's points
-    static_assert((kFlushAt > kMaxWindow ? kFlushAt : kMaxWindow) + kPPT * kBlk <= SLOTS * 9 / 10,
+    // iterations between occupancy checks (each check is a workgroup barrier)
+    constexpr int kCheck = (SLOTS == 1536 && kPPT == 1) ? 2 : 1;
+    static_assert((kFlushAt > kMaxWindow ? kFlushAt : kMaxWindow) + kCheck * kPPT * kBlk <= SLOTS * 9 / 10,
                   "the table must hold the survivors of a flush plus an iteration");
     // entries at a flush: those kept by the previous flush (at most kMaxWindow)
-    // or below the threshold, plus the points of the last iteration
-    constexpr int kMaxEntries = (kFlushAt - 1 > kMaxWindow ? kFlushAt - 1 : (int)kMaxWindow) + kPPT * kBlk;
+    // or below the threshold, plus the points of the iterations since the
+    // last check
+    constexpr int kMaxEntries = (kFlushAt - 1 > kMaxWindow ? kFlushAt - 1 : (int)kMaxWindow) + kCheck * kPPT * kBlk;
     constexpr int kLP = (kMaxEntries + kBT - 1) / kBT;   // list entries per thread
     constexpr int kWords = Acc<MODE>::words;
     extern __shared__ unsigned long long lds[];
@@ -525,6 +528,7 @@ void batch_kernel(BatchArgs a)
             }
         };
         fetch(b_lo, nxt);
+        int since = 0;
         for (int b = b_lo; b < b_hi; b += kPPT) {
             PointT cur[kPPT];
 #pragma unroll
@@ -594,9 +598,11 @@ void batch_kernel(BatchArgs a)
                 }
             }
             append(fresh, std::integral_constant<int, kPPT>());
-            __syncthreads();
             const int64_t next = c + (int64_t)kPPT * kBlk;
-            if (s_occ >= kFlushAt && next < p_end) {
+            if (++since < kCheck || next >= p_end) continue;
+            since = 0;
+            __syncthreads();
+            if (s_occ >= kFlushAt) {
                 // every later point lies in a layer >= the next point's exact
                 // layer index (z order), and a key's layer (key / N^2) is never
                 // below its point's layer: entries with smaller keys are complete
