@@ -18,6 +18,7 @@ public:
     double GetBinContent(int i) const { return _contents.at(i); }
     double GetBinLowEdge(int i) const;
     void AddBinContent(int i, double w) { _contents.at(i) += w; }
+    void SetBinContent(int i, double w) { _contents.at(i) = w; }
     void Add(const Hist1D *other);
     double Integral() const;
     const std::vector<double> &Edges() const { return _edges; }
@@ -30,6 +31,18 @@ private:
     std::vector<double> _edges;
     std::vector<double> _contents;
 };
+
+// Spectrum post-processing of a finished histogram (utils.cc:80-117).  Both
+// loops run over TH1 bins 0..nbins-1, as the reference's do: the underflow bin
+// is included and the last bin is left as it is.
+namespace CPUHistogramUtils {
+// Counts per bin width, divided by the (single-precision) sum of the counts.
+void PMF_to_PDF(Hist1D &h);
+// Density times the bin's mid value: y f(y) for a semilog plot.
+void Prepare_for_Semilog(Hist1D &h);
+// Reads a histogram written by Hist1D::WriteCSV (under- and overflow zero).
+bool ReadCSV(const std::string &path, Hist1D &h);
+}  // namespace CPUHistogramUtils
 
 class Histogram {
 public:

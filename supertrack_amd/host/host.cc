@@ -7,6 +7,7 @@
 #include <cstdlib>
 #include <ctime>
 #include <filesystem>
+#include <fstream>
 #include <iostream>
 #include <sstream>
 #include <thread>
@@ -346,11 +347,67 @@ Hist1D::Hist1D(const std::string &name, const std::vector<double> &edges)
 
 double Hist1D::GetBinLowEdge(int i) const
 {
+    // TAxis::GetBinLowEdge: the edge array inside [1, nbins + 1]; outside it
+    // the average bin width extrapolated from the lower edge
     const int nb = GetNbinsX();
     if (i >= 1 && i <= nb + 1) return _edges[i - 1];
-    if (i <= 0) return _edges[0] - (_edges[1] - _edges[0]);
-    return _edges[nb];
+    return _edges[0] + (double)(i - 1) * ((_edges[nb] - _edges[0]) / (double)nb);
 }
+
+namespace CPUHistogramUtils {
+
+void PMF_to_PDF(Hist1D &h)
+{
+    const int length = h.GetNbinsX();
+    float normalization = 0;
+    for (int i = 0; i < length; ++i) {
+        const double value = h.GetBinContent(i);
+        const double low_edge = h.GetBinLowEdge(i);
+        const double high_edge = h.GetBinLowEdge(i + 1);
+        h.SetBinContent(i, value / (high_edge - low_edge));
+        normalization += value;
+    }
+    for (int i = 0; i < length; ++i) h.SetBinContent(i, h.GetBinContent(i) / normalization);
+}
+
+void Prepare_for_Semilog(Hist1D &h)
+{
+    const int length = h.GetNbinsX();
+    for (int i = 0; i < length; ++i) {
+        const double value = h.GetBinContent(i);
+        const double mid_value = (h.GetBinLowEdge(i + 1) + h.GetBinLowEdge(i)) / 2;
+        h.SetBinContent(i, value * mid_value);
+    }
+}
+
+bool ReadCSV(const std::string &path, Hist1D &h)
+{
+    std::ifstream in(path);
+    if (!in) return false;
+    std::string line, name = "spectrum";
+    std::vector<double> lo, hi, cnt;
+    while (std::getline(in, line)) {
+        if (line.empty()) continue;
+        if (line[0] == '#') {
+            if (name == "spectrum" && line.size() > 2 && line.find(':') == std::string::npos) name = line.substr(2);
+            continue;
+        }
+        if (line.rfind("low_edge", 0) == 0) continue;
+        double a, b, c;
+        if (std::sscanf(line.c_str(), "%lf,%lf,%lf", &a, &b, &c) != 3) return false;
+        lo.push_back(a);
+        hi.push_back(b);
+        cnt.push_back(c);
+    }
+    if (lo.empty()) return false;
+    std::vector<double> edges(lo);
+    edges.push_back(hi.back());
+    h = Hist1D(name, edges);
+    for (size_t i = 0; i < cnt.size(); ++i) h.SetBinContent((int)i + 1, cnt[i]);
+    return true;
+}
+
+}  // namespace CPUHistogramUtils
 
 void Hist1D::Add(const Hist1D *other)
 {

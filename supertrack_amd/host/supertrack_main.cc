@@ -1,5 +1,6 @@
 // supertrack_main.cc -- command line entry (SuperTrack.cu:9-50):
 //   supertrack [macro.ini]   (default ../macros/default.ini)
+//   supertrack --spectrum in.csv out.csv [semilog]
 #include <ctime>
 #include <iostream>
 #include <string>
@@ -26,9 +27,29 @@ static int ini_dump(const std::string &file, const std::string &section, const s
     return 0;
 }
 
+// --spectrum <in.csv> <out.csv> [semilog]: PMF -> PDF (and y f(y)) of a
+// histogram written by a run (CPUHistogramUtils, utils.cc:80-117); no GPU.
+static int spectrum(const std::string &in, const std::string &out, bool semilog)
+{
+    Hist1D h;
+    if (!CPUHistogramUtils::ReadCSV(in, h)) {
+        std::cerr << "Cannot read histogram " << in << std::endl;
+        return 1;
+    }
+    CPUHistogramUtils::PMF_to_PDF(h);
+    if (semilog) CPUHistogramUtils::Prepare_for_Semilog(h);
+    if (!h.WriteCSV(out, {{"spectrum", semilog ? "y f(y)" : "f(y)"}})) {
+        std::cerr << "Cannot write " << out << std::endl;
+        return 1;
+    }
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
     if (argc == 5 && std::string(argv[1]) == "--ini-dump") return ini_dump(argv[2], argv[3], argv[4]);
+    if ((argc == 4 || argc == 5) && std::string(argv[1]) == "--spectrum")
+        return spectrum(argv[2], argv[3], argc == 5 && std::string(argv[4]) == "semilog");
     time_t start = time(nullptr);
     std::string path = "../macros/default.ini";
     if (argc == 2) path = argv[1];
