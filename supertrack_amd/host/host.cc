@@ -8,6 +8,7 @@
 #include <ctime>
 #include <filesystem>
 #include <fstream>
+#include <future>
 #include <iostream>
 #include <sstream>
 #include <thread>
@@ -223,6 +224,23 @@ void Track::AllocateAndLoadTrack(const ThreadTask &task)
     y = c.y;
     z = c.z;
     edep = c.edep;
+}
+
+void Track::LoadFromHost(const double *hx, const double *hy, const double *hz, const double *he, int64_t n,
+                         void *stream)
+{
+    if (n > c.capacity) fatal("Track::LoadFromHost: track larger than the allocated capacity");
+    const hipStream_t s = (hipStream_t)stream;
+    const size_t bytes = (size_t)n * sizeof(double);
+    bool ok = true;
+    if (n > 0) {
+        ok = ok && hipMemcpyAsync(c.x, hx, bytes, hipMemcpyHostToDevice, s) == hipSuccess;
+        ok = ok && hipMemcpyAsync(c.y, hy, bytes, hipMemcpyHostToDevice, s) == hipSuccess;
+        ok = ok && hipMemcpyAsync(c.z, hz, bytes, hipMemcpyHostToDevice, s) == hipSuccess;
+        ok = ok && hipMemcpyAsync(c.edep, he, bytes, hipMemcpyHostToDevice, s) == hipSuccess;
+    }
+    if (!ok) fatal("Track::LoadFromHost: copy to the device failed");
+    c.n = n;
 }
 
 void Track::LoadTrack(const ThreadTask &task, void *stream)
@@ -556,6 +574,7 @@ void SuperTrackManager::RunPerTrack()
     std::vector<Hist1D> outputs(nw);
     auto work = [&](size_t k) {
         const ThreadAllocation &alloc = _threadAllocations[k];
+        const std::vector<ThreadTask> &tasks = alloc.GetTasks();
         hipStream_t stream = nullptr;
         if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) fatal("hipStreamCreate failed");
         Histogram histogram(*_inputFileReader);
@@ -563,15 +582,37 @@ void SuperTrackManager::RunPerTrack()
         SimulationMethod *method = SimulationMethodFactory::GetInstance().Construct(*_inputFileReader);
         method->SetStream(stream);
         int64_t cap = 1;
-        for (const ThreadTask &task : alloc.GetTasks())
-            cap = std::max<int64_t>(cap, task.GetExitPoint() - task.GetEntryPoint());
+        for (const ThreadTask &task : tasks) cap = std::max<int64_t>(cap, task.GetExitPoint() - task.GetEntryPoint());
         Track track;
         track.AllocateEmptyTrack((int)cap);
         VolumeEdepPair edepsInTarget;
         edepsInTarget.Allocate((uint64_t)cap);
-        for (ThreadTask task : alloc.GetTasks()) {
+        // Ingest: the next task's entries are read from its file into one of
+        // two pinned host buffers while the current task runs on the GPU.
+        double *pinned[2] = {nullptr, nullptr};
+        for (double *&p : pinned)
+            if (hipHostMalloc((void **)&p, 4 * (size_t)cap * sizeof(double), hipHostMallocDefault) != hipSuccess)
+                fatal("hipHostMalloc failed");
+        auto read_task = [&](size_t j) {
+            const ThreadTask &t = tasks[j];
+            double *b = pinned[j & 1];
+            TrackFile::Header h;
+            if (!TrackFile::ReadHeader(t.GetFilename(), h) ||
+                !TrackFile::ReadRange(t.GetFilename(), h, t.GetEntryPoint(), t.GetExitPoint(), b, b + cap,
+                                      b + 2 * cap, b + 3 * cap))
+                fatal("Cannot read track entries from " + t.GetFilename());
+        };
+        std::future<void> next;
+        if (!tasks.empty()) next = std::async(std::launch::async, read_task, 0);
+        for (size_t j = 0; j < tasks.size(); ++j) {
+            const ThreadTask &task = tasks[j];
+            next.get();
+            // the buffer of task j + 1 was last read by the copy of task j - 1,
+            // which the stream synchronisation at the end of that task completed
+            if (j + 1 < tasks.size()) next = std::async(std::launch::async, read_task, j + 1);
             const int64_t n = task.GetExitPoint() - task.GetEntryPoint();
-            track.LoadTrack(task, stream);
+            const double *b = pinned[j & 1];
+            track.LoadFromHost(b, b + cap, b + 2 * cap, b + 3 * cap, n, stream);
             edepsInTarget.c.capacity = std::max<int64_t>(n, 1);
             method->AllocateTrackProcess(track, task);
             histogram.AllocateTrackProcess(edepsInTarget);
@@ -588,6 +629,7 @@ void SuperTrackManager::RunPerTrack()
         outputs[k] = histogram.GetCPUHistogram();
         track.Free();
         edepsInTarget.Free();
+        for (double *p : pinned) (void)hipHostFree(p);
         (void)hipStreamDestroy(stream);
     };
     std::vector<std::thread> threads;
