@@ -59,6 +59,8 @@ def parse():
     p.add_argument("--autotune", action="store_true")
     p.add_argument("--workers", type=int, default=4,
                    help="plain path: concurrent workers, one HIP stream each (the reference's [Run] Threads)")
+    p.add_argument("--via-cli", action="store_true",
+                   help="plain workloads: also time the C++ driver (supertrack CLI) on the same tracks")
     p.add_argument("--copy-probe", action="store_true",
                    help="only run a 1 GiB device-to-device copy 5 times (calibrates HBM byte counters)")
     return p.parse_args()
@@ -283,6 +285,13 @@ def plain_main(args, w, n_tracks, n_os):
     secs = float(np.mean(times))
     value = deposits / secs
     log(f"{args.workload} plain: {secs * 1e3:.1f} ms/step, {value:.3e} deposits/s")
+    extra = {}
+    if args.via_cli:
+        ms = time_cli(w, host, n_os, args.workers)
+        extra = {"cli_run_ms": ms, "cli_deposits_per_s": deposits / (ms * 1e-3),
+                 "cli_note": f"supertrack CLI, {args.workers} workers: track file reads (prefetched), "
+                             "per-track loops and the output file, in the program's own clock"}
+        log(f"{args.workload} plain via CLI: {ms:.1f} ms, {deposits / (ms * 1e-3):.3e} deposits/s")
     cpu = None
     if not args.no_cpu:
         rate, threads, ns, cdep, csecs = cpu_baseline(w, 1, n_os)
@@ -302,7 +311,61 @@ def plain_main(args, w, n_tracks, n_os):
                      "basis": "32 B per deposit / host-clock step time (per-track launch sequences)"},
         "cpu_baseline": cpu,
     }
+    out.update(extra)
     print(json.dumps(out))
+
+
+def time_cli(w, host, n_os, workers):
+    """Runs the C++ driver (supertrack CLI) on the workload's tracks written
+    as a track file; returns the run time it reports (ms)."""
+    import subprocess
+    import tempfile
+    from supertrack_amd import trackfile
+    x, y, z, e, off, gt = host
+    (H, D), = w["geometries"]
+    kind, nb, lo, hi = w["hists"][0]
+    cli = os.path.join(ROOT, "supertrack_amd", "supertrack")
+    with tempfile.TemporaryDirectory() as tmp:
+        os.makedirs(os.path.join(tmp, "tracks"))
+        os.makedirs(os.path.join(tmp, "out"))
+        trackfile.write(os.path.join(tmp, "tracks", "tracks.sttrk"), x, y, z, e, off)
+        macro = os.path.join(tmp, "bench.ini")
+        with open(macro, "w") as f:
+            f.write(f"""[Run]
+Type = Folder
+Path = {tmp}/tracks
+FirstFile = -1
+LastFile = -1
+Threads = {workers}
+Oversamples = {n_os}
+RandomSeed = 1234
+
+[Simulation]
+Method = VoxelConstrainedSphere
+
+[VoxelConstrainedSphere]
+ScoringRegionHalfLength = {H}
+ScoringSphereDiameter = {D}
+ShiftTrack = true
+ShiftTrackZ = true
+
+[Histogram]
+Type = {kind}
+NBins = {nb}
+BinLower = {lo}
+BinUpper = {hi}
+
+[Output]
+Path = {tmp}/out/
+NamePrefix = bench_
+""")
+        res = subprocess.run([cli, macro], capture_output=True, text=True, timeout=600)
+        if res.returncode != 0:
+            raise RuntimeError(res.stdout + res.stderr)
+        for line in res.stdout.splitlines():
+            if line.startswith("Run time:"):
+                return float(line.split()[2])
+    raise RuntimeError("the CLI printed no run time")
 
 
 def main():

@@ -1,6 +1,7 @@
 // supertrack_main.cc -- command line entry (SuperTrack.cu:9-50):
 //   supertrack [macro.ini]   (default ../macros/default.ini)
 //   supertrack --spectrum in.csv out.csv [semilog]
+#include <chrono>
 #include <ctime>
 #include <iostream>
 #include <string>
@@ -56,9 +57,12 @@ int main(int argc, char **argv)
     SuperTrackManager &manager = SuperTrackManager::GetInstance();
     manager.AddSimulationMethod("VoxelConstrainedSphere", &VoxelConstrainedSphereMethod::Construct);
     manager.Initialize(new INIReader(path));
+    const auto t0 = std::chrono::steady_clock::now();
     manager.Run();
+    const auto t1 = std::chrono::steady_clock::now();
     const Hist1D &out = manager.GetOutput();
     std::cout << "Scored spheres: " << out.Integral() << std::endl;
+    std::cout << "Run time: " << std::chrono::duration<double, std::milli>(t1 - t0).count() << " ms" << std::endl;
     std::cout << "SuperTrack concluding after: " << (time(nullptr) - start) << " seconds." << std::endl;
     return 0;
 }
