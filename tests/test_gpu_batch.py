@@ -272,3 +272,46 @@ def test_batch_points_on_decision_boundaries(precision, diameter):
         assert info["fp64_rechecks"] > 0.2 * n_tracks * n_os * per_os
     else:
         assert info["fp64_rechecks"] == 0
+
+
+@pytest.mark.parametrize("precision", [st.ST_F64, st.ST_F32])
+def test_batch_ragged_library_with_empty_and_tiny_tracks(precision):
+    """Empty tracks, one-point tracks and tracks of 255/256/257 points (block
+    edges) mixed with ordinary ones: counts and records equal the oracle's."""
+    w = W.workload("cfg2")
+    x, y, z, e, off, _ = W.make_library(w, n_tracks=6)
+    rng = np.random.default_rng(5)
+    pieces = []
+    for t in range(6):
+        b, en = off[t], off[t + 1]
+        pieces.append((x[b:en], y[b:en], z[b:en], e[b:en]))
+    sizes = [0, 1, 255, 256, 257, 0, 513]
+    orig = list(pieces)
+    for k, n in enumerate(sizes):
+        src = orig[k % 6]
+        sel = rng.choice(len(src[0]), size=n, replace=False) if n else np.zeros(0, np.int64)
+        pieces.insert(2 * k, tuple(a[sel] for a in src))
+    xs, ys, zs, es = (np.concatenate([p[i] for p in pieces]) for i in range(4))
+    offs = np.concatenate([[0], np.cumsum([len(p[0]) for p in pieces])]).astype(np.int64)
+    geoms = [po.geometry(2.5e3, 10.0)]
+    oracle = oracle_segments(xs, ys, zs, es, offs, geoms, None, 7, 21, True, True)
+    counts, rec, info, edges = gpu_run(xs, ys, zs, es, offs, [(2.5e3, 10.0)], None, 7, 21, True, True, precision,
+                                       [("lin", 3000, 0.01, 300.0)], record=400000)
+    assert info["n_tracks"] == len(pieces) and info["exact_tracks"] == 0
+    rtol = 1e-12 if precision == st.ST_F64 else 1e-6
+    n, _ = compare_records(rec, oracle, rtol)
+    assert n > 100
+    check_bins(counts, edges, oracle, rtol)
+
+
+def test_batch_empty_library():
+    geoms = [st.SphericalGeometry(2.5e3, 10.0)]
+    z0 = np.zeros(0)
+    lib = st.TrackLibrary(z0, z0, z0, z0, np.zeros(1, np.int64), geoms, None, st.ST_F32)
+    h = st.Histogram(100, 0.1, 10.0, "lin")
+    b = st.BatchRun(lib, [h], 10, 1, True, True)
+    b.run()
+    b.sync()
+    assert h.fetch().sum() == 0
+    assert b.stats()["points_scored"] == 0
+    b.free(); h.free(); lib.free()
