@@ -296,14 +296,14 @@ __device__ int wave_lower_bound(int n, Pred pred)
 template <int MODE, int SLOTS, int kPPT>
 constexpr int waves_per_eu()
 {
-    return (MODE == ST_F32 && SLOTS == 1536 && kPPT == 1) ? 5 : 1;
+    return (MODE == ST_F32 && (SLOTS == 1280 || SLOTS == 1536) && kPPT == 1) ? 5 : 1;
 }
 
 template <int MODE, int SLOTS, int kPPT>
 __global__ __launch_bounds__(kBT) __attribute__((amdgpu_waves_per_eu(waves_per_eu<MODE, SLOTS, kPPT>())))
 void batch_kernel(BatchArgs a)
 {
-    static_assert(SLOTS == 1536 || SLOTS == 2048 || SLOTS == 4096, "table sizes");
+    static_assert(SLOTS == 1280 || SLOTS == 1536 || SLOTS == 2048 || SLOTS == 4096, "table sizes");
     // occupancy never exceeds max(half table, survivors) + one iteration's points
     // A flush starts once the table is a quarter full: a low load factor
     // keeps the linear-probe chains (dependent LDS round trips, paid by the
@@ -679,6 +679,8 @@ const void *kernel_ptr()
 template <int MODE>
 const void *pick_kernel_mode(int slots, int ppt)
 {
+    // 1280 slots hold the largest survivor set plus one point per thread only
+    if (slots == 1280) return ppt == 2 ? nullptr : kernel_ptr<MODE, 1280, 1>();
     if (slots == 1536) return ppt == 2 ? kernel_ptr<MODE, 1536, 2>() : kernel_ptr<MODE, 1536, 1>();
     if (slots == 2048) return ppt == 2 ? kernel_ptr<MODE, 2048, 2>() : kernel_ptr<MODE, 2048, 1>();
     return ppt == 2 ? kernel_ptr<MODE, 4096, 2>() : kernel_ptr<MODE, 4096, 1>();
@@ -861,6 +863,7 @@ int configure(st_batch *b, int slots, int bpc, int ppt)
     if (lds > 160 * 1024)
         return set_error(ST_ERANGE, "batched kernel needs %zu B of LDS (limit 160 KiB)", lds);
     const void *k = pick_kernel(b->lib->precision, slots, b->ppt);
+    if (!k) return set_error(ST_EINVAL, "batched kernel: %d table slots take one point per thread only", slots);
     ST_HIP_TRY(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     int per_cu = 0;
     ST_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, kBT, lds));
@@ -1185,22 +1188,31 @@ int st_batch_create(st_library *lib, const st_batch_opts *opts, st_hist *const *
     // defaults: the fastest shapes measured on MI355X for 2 nm and 10 nm spheres
     const bool f32 = lib->precision == ST_F32;
     int slots = opts->table_slots ? opts->table_slots : 1536;
-    if (slots != 1536 && slots != 2048 && slots != 4096)
-        return fail(set_error(ST_EINVAL, "st_batch_create: table_slots must be 1536, 2048 or 4096"));
+    if (slots != 1280 && slots != 1536 && slots != 2048 && slots != 4096)
+        return fail(set_error(ST_EINVAL, "st_batch_create: table_slots must be 1280, 1536, 2048 or 4096"));
     int bpc = opts->blocks_per_cu ? opts->blocks_per_cu : (f32 ? 5 : 3);
     const int ppt = opts->points_per_thread ? opts->points_per_thread : (f32 ? 1 : 2);
     if (ppt != 1 && ppt != 2) return fail(set_error(ST_EINVAL, "st_batch_create: points_per_thread must be 1 or 2"));
     rc = configure(b, slots, bpc, ppt);
     if (rc) return fail(rc);
     // FP32 with fewer than five workgroups per CU (large privatised
-    // histograms): two points per thread hide more latency
+    // histograms): a 1280-slot table if it fits more workgroups, else two
+    // points per thread (both measured faster on MI355X in that case)
     if (f32 && !opts->points_per_thread && b->blocks_per_cu < 5) {
-        rc = configure(b, slots, bpc, 2);
-        if (rc) return fail(rc);
+        const int fit1536 = b->blocks_per_cu;
+        bool done = false;
+        if (!opts->table_slots) {
+            rc = configure(b, 1280, bpc, 1);
+            done = rc == ST_OK && b->blocks_per_cu > fit1536;
+        }
+        if (!done) {
+            rc = configure(b, slots, bpc, 2);
+            if (rc) return fail(rc);
+        }
     }
     if (opts->autotune && !lib->fast_order.empty() && opts->n_oversamples > 0) {
         // time each candidate on one full pass; keep the fastest
-        const int cand_slots[] = {1536, 2048, 4096};
+        const int cand_slots[] = {1280, 1536, 2048, 4096};
         const int cand_bpc[] = {2, 3, 4, 5, 6, 8};
         hipEvent_t e0, e1;
         ST_HIP_TRY(hipEventCreate(&e0));
