@@ -177,7 +177,7 @@ def test_batch_is_deterministic_and_tuning_invariant():
     x, y, z, e, off = st.synth_library(spec, 24, 50)
     results = []
     for slots, bpc, graph, ppt in [(1536, 5, True, 1), (2048, 2, False, 2), (4096, 1, True, 2), (1536, 3, False, 2),
-                                   (2048, 4, True, 1)]:
+                                   (2048, 4, True, 1), (1280, 4, False, 1)]:
         counts, _, _, _ = gpu_run(x, y, z, e, off, [(2.5e3, 10.0)], None, 8, 3, True, True, st.ST_F32,
                                   [("log", 1000, 1e-2, 1e3)], table_slots=slots, blocks_per_cu=bpc, use_graph=graph,
                                   points_per_thread=ppt)
