@@ -103,8 +103,9 @@ typedef struct {
     int32_t *num_elements;   /* device counter, written by process_oversample */
     int64_t capacity;
     uint64_t key_bound;      /* largest key the producer can write; 0 = unknown
-                                (set by st_vcs_process_oversample; the histogram
-                                sorts only the bits this needs) */
+                                (set by st_vcs_process_oversample; the radix
+                                sort for more than 12288 pairs runs only the
+                                passes this needs) */
 } st_pairs;
 
 int st_pairs_alloc(st_pairs *p, int64_t capacity);
