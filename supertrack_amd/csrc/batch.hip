@@ -1183,9 +1183,10 @@ int st_batch_create(st_library *lib, const st_batch_opts *opts, st_hist *const *
         st_vcs_set_stream(m, b->stream);
         b->methods.push_back(m);
     }
-    // every table size holds the largest survivor set (kMaxWindow) or a half
-    // table, plus an iteration's points, so every fast-path track fits
-    // defaults: the fastest shapes measured on MI355X for 2 nm and 10 nm spheres
+    // every table size holds the largest survivor set (kMaxWindow) or a
+    // quarter table, plus the points of the iterations between occupancy
+    // checks, so every fast-path track fits.  Defaults: the fastest shapes
+    // measured on MI355X for 2 nm and 10 nm spheres.
     const bool f32 = lib->precision == ST_F32;
     int slots = opts->table_slots ? opts->table_slots : 1536;
     if (slots != 1280 && slots != 1536 && slots != 2048 && slots != 4096)
