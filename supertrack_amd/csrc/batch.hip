@@ -82,6 +82,9 @@ struct GeomDev {
     double yscale;      // denom_inv * 2^32 (ST_F32 fixed point)
     double qc;          // gso / D: lattice units of the voxel centre offset
     double r2q;         // r^2 / D^2
+    // ST_F64 thresholds in lattice units (as BlockF32's, with the FP64
+    // rounding bound dl of the geometry: the data carry no FP32 rounding)
+    double hq_in, hq_out, fm, mrgq;
     unsigned long long N, N2;
 };
 
@@ -547,10 +550,26 @@ void batch_kernel(BatchArgs a)
                 unsigned long long key = 0;
                 long long v0 = 0, v1 = 0;
                 if constexpr (MODE == ST_F64) {
-                    if (valid) {
-                        in = exact_member(cur[j].x, cur[j].y, cur[j].z + g.H, g, sh.x, sh.y, sh.z, key);
-                        if (in) Acc<ST_F64>::split(cur[j].w / g.denom, v0, v1);
-                    }
+                    // lattice units straight from the FP64 data; decisions
+                    // within the rounding bound are redone exactly
+                    const double qx = fma(cur[j].x, g.invD, sqx - g.qc);
+                    const double qy = fma(cur[j].y, g.invD, sqy - g.qc);
+                    const double qz = fma(cur[j].z, g.invD, (sqz + g.H * g.invD) - g.qc);
+                    const double px = fabs(qx + g.qc), py = fabs(qy + g.qc), pz = fabs(qz + g.qc);
+                    const bool box_in = px < g.hq_in && py < g.hq_in && pz < g.hq_in;
+                    const bool box_out = px > g.hq_out || py > g.hq_out || pz > g.hq_out;
+                    const double tx = qx + kRoundMagic, ty = qy + kRoundMagic, tz = qz + kRoundMagic;
+                    const double ux = (tx - kRoundMagic) - qx, uy = (ty - kRoundMagic) - qy,
+                                 uz = (tz - kRoundMagic) - qz;
+                    const double distq = fma(ux, ux, fma(uy, uy, uz * uz));
+                    const bool near = fabs(ux) >= g.fm || fabs(uy) >= g.fm || fabs(uz) >= g.fm ||
+                                      fabs(distq - g.r2q) <= g.mrgq;
+                    const bool amb = valid && !box_out && (!box_in || near);
+                    in = valid && box_in && !near && distq < g.r2q;
+                    key = lattice_key((unsigned)__double2loint(tx), (unsigned)__double2loint(ty),
+                                      (unsigned)__double2loint(tz), (unsigned)g.N);
+                    if (amb) in = exact_member(cur[j].x, cur[j].y, cur[j].z + g.H, g, sh.x, sh.y, sh.z, key);
+                    if (in) Acc<ST_F64>::split(cur[j].w / g.denom, v0, v1);
                 } else {
                     // block-uniform terms (scalar loads)
                     const BlockF32 bk = load_block(a.blocks, td.block0 + b + j);
@@ -1093,6 +1112,14 @@ int st_library_create(const double *x, const double *y, const double *z, const d
         gd[i].yscale = gd[i].denom_inv * kTwo32;
         gd[i].qc = g.gso / g.diameter;
         gd[i].r2q = (g.radius * g.radius) / (g.diameter * g.diameter);
+        {
+            const double dl = kTwoM44 * (7.0 * g.half_length + std::fabs(g.gso) + g.diameter + 16.0);
+            const double dlq = dl / g.diameter;
+            gd[i].hq_in = (g.half_length - dl) / g.diameter;
+            gd[i].hq_out = (g.half_length + dl) / g.diameter;
+            gd[i].fm = 0.5 - (dlq + 1e-12);
+            gd[i].mrgq = 3.0 * dlq * (1.0 + 2.0 * dlq) + 1e-14 * gd[i].r2q;
+        }
         gd[i].N = (unsigned long long)g.n_linear;
         gd[i].N2 = gd[i].N * gd[i].N;
     }
