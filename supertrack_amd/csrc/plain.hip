@@ -350,25 +350,64 @@ __device__ void lds_sort_reduce_bin(const HistArgs &a, int m, unsigned char *sme
     // reduce-by-key left to right (Histogram.cu:153), bin (:158): the pair at
     // sorted position i starts its key's sum if no earlier pair of its bucket
     // has the same key; the sum then takes the bucket's pairs of that key in
-    // order (ascending pair index: the sort is stable)
-    for (int i = tid; i < m; i += kSortThreads) {
+    // order (ascending pair index: the sort is stable).  A thread's positions
+    // are handled together, stage by stage, so the global loads of its pairs
+    // (keys, energies, bin edges) overlap instead of queueing behind each other.
+    constexpr int kR = (kLdsCap + kSortThreads - 1) / kSortThreads;
+    uint64_t kk[kR];
+    double s[kR];
+    bool lead[kR];
+#pragma unroll
+    for (int r = 0; r < kR; ++r) {
+        const int i = tid + r * kSortThreads;
+        lead[r] = false;
+        if (i < m) {
+            const int pi = src_v[i];
+            kk[r] = a.keys[pi];
+            s[r] = a.y[pi];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < kR; ++r) {
+        const int i = tid + r * kSortThreads;
+        if (i >= m) continue;
         const uint16_t hb = src_h[i];
-        const int pi = src_v[i];
-        const uint64_t kk = a.keys[pi];
         bool first = true;
         for (int j = i - 1; j >= 0 && src_h[j] == hb; --j) {
-            if (a.keys[src_v[j]] == kk) {
+            if (a.keys[src_v[j]] == kk[r]) {
                 first = false;
                 break;
             }
         }
         if (!first) continue;
-        double s = a.y[pi];
+        lead[r] = true;
+        double acc = s[r];
         for (int j = i + 1; j < m && src_h[j] == hb; ++j) {
             const int pj = src_v[j];
-            if (a.keys[pj] == kk) s = s + a.y[pj];
+            if (a.keys[pj] == kk[r]) acc = acc + a.y[pj];
         }
-        bin_sum(a, s);
+        s[r] = acc;
+    }
+    // binning: guesses and edge loads for all of the thread's sums first
+    int bb[kR];
+    double lo[kR], hi[kR];
+#pragma unroll
+    for (int r = 0; r < kR; ++r) {
+        const double v = lead[r] ? s[r] : a.guess.e0;
+        float t;
+        if (a.guess.type == ST_HIST_LIN) t = (float)((v - a.guess.origin) * a.guess.inv_width);
+        else t = (__log2f((float)v) - a.guess.log2_origin) * a.guess.log2_inv_width;
+        bb[r] = (t > 0.0f) ? (int)fminf(t, (float)(a.guess.nbins - 1)) : 0;
+        lo[r] = a.edges[bb[r]];
+        hi[r] = a.edges[bb[r] + 1];
+    }
+#pragma unroll
+    for (int r = 0; r < kR; ++r) {
+        if (!lead[r]) continue;
+        const double v = s[r];
+        int b = bb[r];
+        if (!(v >= lo[r] && v < hi[r])) b = bin_fast(a.edges, a.guess, v);
+        if (b >= 0) atomicAdd(&a.bins[b], 1ull);
     }
 }
 
