@@ -25,8 +25,9 @@
 // parameters).  When the table fills past a quarter, every entry whose key lies
 // below the first layer still to come is complete: those are converted to
 // lineal energy and binned into LDS-privatised histograms; the rest are kept.
-// Tracks whose densest 4-layer window could hold more spheres than the table
-// keeps (or with non-finite data) are scored by the exact per-segment path.
+// Tracks whose densest z-window two diameters wide could touch more spheres
+// than a flush may keep (768), with non-finite data or beyond the fixed-point
+// headroom are scored by the exact per-segment path.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
