@@ -381,7 +381,7 @@ void batch_kernel(BatchArgs a)
     // re-inserted into the cleared table.  Threads walk the dense list of
     // used slots, so the work follows the entries, not the table size, and
     // the bin edges of two entries are loaded before either is compared.
-    auto flush = [&](unsigned long long bound, int64_t seg) {
+    auto flush = [&](unsigned long long bound) {
         __syncthreads();
         int n = uniform_i32(s_occ);
         if (n > kLP * kBT) {   // cannot happen (the bound above); reported, never read past the list
@@ -417,7 +417,9 @@ void batch_kernel(BatchArgs a)
                     if (em[u] && a.rec_cap > 0) {
                         const unsigned long long p = atomicAdd(a.rec_count, 1ull);
                         if ((int64_t)p < a.rec_cap) {
-                            a.rec_seg[p] = seg;
+                            // test hook only: the segment id from the ticket
+                            const long long tk = s_seg;
+                            a.rec_seg[p] = (int64_t)a.order[tk / a.n_os] * a.n_os + tk % a.n_os;
                             a.rec_key[p] = k;
                             a.rec_sum[p] = vv[u];
                         }
@@ -489,7 +491,6 @@ void batch_kernel(BatchArgs a)
         const GeomDev g = a.geoms[td.geom];
         const Shift sh = shift_for(a.seed, a.track_index_base + (uint32_t)t, (uint32_t)os, a.shift_xy,
                                    a.shift_z, g.L, g.H);
-        const int64_t seg_id = (int64_t)t * a.n_os + os;
         // the shift in lattice units (ST_F32 fast path)
         const double sqx = sh.x * g.invD, sqy = sh.y * g.invD, sqz = sh.z * g.invD;
         // x/y bounds cull: fl(x + s) is monotone in x
@@ -630,10 +631,10 @@ void batch_kernel(BatchArgs a)
                 double lb = nearest_index(zs - g.gso, g.D, g.invD);
                 if (!(lb > 0.0)) lb = 0.0;
                 const unsigned long long bound = (unsigned long long)lb * g.N2;
-                flush(bound, seg_id);
+                flush(bound);
             }
         }
-        flush(kEmpty, seg_id);
+        flush(kEmpty);
         if (a.hist_lds_words && binned > 0x40000000ll) {
             // keep the 32-bit LDS counters far from overflow
 #pragma unroll
