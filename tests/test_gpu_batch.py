@@ -315,3 +315,32 @@ def test_batch_empty_library():
     assert h.fetch().sum() == 0
     assert b.stats()["points_scored"] == 0
     b.free(); h.free(); lib.free()
+
+
+@pytest.mark.parametrize("precision", [st.ST_F64, st.ST_F32])
+@pytest.mark.parametrize("H,D", [(1.0, 10.0), (5e3, 0.04)])
+def test_batch_degenerate_and_fine_lattices(precision, H, D):
+    """N = 1 (sphere larger than the voxel) and N = 250000 (keys above 2^53)."""
+    g = po.geometry(H, D)
+    rng = np.random.default_rng(int(H) + precision)
+    Hp = g.half_length
+    xs, ys, zs, es, off = [], [], [], [], [0]
+    for t in range(5):
+        n = 5000
+        x = rng.uniform(-1.2 * Hp, 1.2 * Hp, n)
+        y = rng.uniform(-1.2 * Hp, 1.2 * Hp, n)
+        z = np.sort(rng.uniform(-1.2 * Hp, 1.2 * Hp, n)) - Hp
+        if D < 1:
+            x = np.round(x / D) * D + rng.normal(0, 0.2 * D, n)
+        xs.append(x); ys.append(y); zs.append(z); es.append(rng.uniform(1, 1000, n))
+        off.append(off[-1] + n)
+    x, y, z, e = (np.concatenate(a) for a in (xs, ys, zs, es))
+    off = np.array(off, np.int64)
+    oracle = oracle_segments(x, y, z, e, off, [g], None, 4, 13, True, True)
+    counts, rec, info, edges = gpu_run(x, y, z, e, off, [(H, D)], None, 4, 13, True, True, precision,
+                                       [("log", 1000, 1e-2, 1e5)], record=400000)
+    rtol = 1e-12 if precision == st.ST_F64 else 1e-6
+    if info["exact_tracks"] == 0:
+        n, _ = compare_records(rec, oracle, rtol)
+        assert n > 0
+    check_bins(counts, edges, oracle, rtol)

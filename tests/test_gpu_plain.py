@@ -260,3 +260,22 @@ def test_plain_concurrent_workers_on_streams_match_oracle():
     bins, _, _ = po.run(x, y, z, e, off, [po.geometry(2.5e3, 10.0)], None, 20, W.SEED, True, True,
                         [po.edges(kind, lo, hi, nb)])
     np.testing.assert_array_equal(sum(hs), bins[0].astype(np.float64))
+
+
+@pytest.mark.parametrize("H,D", [(1.0, 10.0), (5e3, 0.04)])
+def test_pairs_degenerate_and_fine_lattices(H, D):
+    """The sphere-larger-than-voxel branch (N = 1) and a lattice with keys
+    above 2^53 (N = 250000): pairs equal the oracle's."""
+    g = po.geometry(H, D)
+    rng = np.random.default_rng(int(H))
+    n = 20000
+    Hp = g.half_length
+    x = rng.uniform(-1.2 * Hp, 1.2 * Hp, n)
+    y = rng.uniform(-1.2 * Hp, 1.2 * Hp, n)
+    z = rng.uniform(-1.2 * Hp, 1.2 * Hp, n) - Hp
+    if D < 1:   # concentrate points so that spheres collect several
+        x = np.round(x / D) * D + rng.normal(0, 0.2 * D, n)
+        y = np.round(y / D) * D + rng.normal(0, 0.2 * D, n)
+    e = rng.uniform(1, 1000, n)
+    total = assert_pairs_equal(x, y, z, e, H, D, 8, 3, 3, True, True)
+    assert total > 100
