@@ -226,24 +226,34 @@ __global__ __launch_bounds__(kScoreBlock) void score_kernel(ScoreArgs a)
 // Histogram: stable LSD radix sort -> left-to-right reduce-by-key -> bin
 // ---------------------------------------------------------------------------
 // Small inputs (m <= kLdsCap pairs) are grouped by one 1024-thread workgroup
-// in LDS: a stable counting sort on a 12-bit hash of the key (three 4-bit
-// passes, whatever the key width) puts every key's pairs in one bucket, in
-// pair order; the reduce then sums each key's pairs left to right within its
-// bucket.  Larger inputs take the onesweep kernels below (8-bit digits over
-// the full keys).
+// in LDS: one stable counting-sort pass on a 10-bit hash of the key puts every
+// key's pairs in one bucket, in pair order; the reduce then sums each key's
+// pairs left to right within its bucket, comparing keys staged in LDS in
+// sorted order.  Larger inputs take the onesweep kernels below (8-bit digits
+// over the full keys).
 constexpr int kSortThreads = 1024;
 constexpr int kSortWaves = kSortThreads / 64;
 constexpr int kLdsCap = 12288;
-constexpr int kDigitBits = 4;
-constexpr int kDigits = 1 << kDigitBits;
-constexpr int kHashBits = 12;
-constexpr int kHashPasses = kHashBits / kDigitBits;
-constexpr size_t kLdsBytes = (size_t)kLdsCap * 4 * sizeof(uint16_t) +
-                             (size_t)kDigits * kSortThreads * sizeof(uint16_t);
+constexpr int kHashBits = 10;
+constexpr int kBuckets = 1 << kHashBits;
+// per (bucket, wave) counts, bucket-major: thread t scans bucket t's row
+constexpr size_t kCntBytes = (size_t)kBuckets * kSortWaves * sizeof(uint16_t);
+// layout [P hash][P index][counts][Q hash][Q index]: the sorted result lands
+// in Q, and P + counts is one contiguous free region
+constexpr size_t kSortBytes = (size_t)kLdsCap * 4 * sizeof(uint16_t) + kCntBytes;
+// free region after the sort: keys and energies in sorted order fit up to
+// kStage pairs
+constexpr int kStage = (int)((kLdsCap * 2 * sizeof(uint16_t) + kCntBytes) / (2 * sizeof(uint64_t)));
+// histograms of up to kBinCap bins are counted in LDS and added to the global
+// counters once per launch: the lineal energies of one track fall in few
+// bins, and global atomics on one address serialise in L2
+constexpr int kBinCap = 4096;
+constexpr size_t kLdsBytes = kSortBytes + (size_t)kBinCap * sizeof(uint32_t);
+static_assert(kBuckets == kSortThreads, "one bucket row per thread in the scan");
 
-__device__ __forceinline__ uint16_t key_bucket(uint64_t k)
+__device__ __forceinline__ uint16_t key_bucket(uint64_t k, int bits)
 {
-    return (uint16_t)((k * 0x9E3779B97F4A7C15ull) >> (64 - kHashBits));
+    return (uint16_t)((k * 0x9E3779B97F4A7C15ull) >> (64 - bits));
 }
 struct HistArgs {
     const uint64_t *keys;
@@ -289,125 +299,185 @@ __device__ __forceinline__ void bin_sum(const HistArgs &a, double s)
     if (b >= 0) atomicAdd(&a.bins[b], 1ull);
 }
 
+__device__ __forceinline__ void bin_sum_lds(const HistArgs &a, uint32_t *sbins, double s)
+{
+    const int b = bin_fast(a.edges, a.guess, s);
+    if (b >= 0) atomicAdd(&sbins[b], 1u);
+}
+
+// Lanes of the wave whose valid flag is set and whose hash equals this lane's.
+__device__ __forceinline__ uint64_t hash_peers(uint32_t h, bool valid)
+{
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < kHashBits; ++b) {
+        const bool bit = (h >> b) & 1u;
+        const uint64_t bb = __ballot(bit);
+        peers &= bit ? bb : ~bb;
+    }
+    return peers;
+}
+
 // One workgroup: group m <= kLdsCap pairs by key bucket in LDS, reduce, bin.
 __device__ void lds_sort_reduce_bin(const HistArgs &a, int m, unsigned char *smem)
 {
-    uint16_t *h0 = reinterpret_cast<uint16_t *>(smem);
-    uint16_t *h1 = h0 + kLdsCap;
-    uint16_t *v0 = h1 + kLdsCap;
-    uint16_t *v1 = v0 + kLdsCap;
-    uint16_t *cnt = v1 + kLdsCap;   // [kDigits][kSortThreads]
+    uint16_t *ph = reinterpret_cast<uint16_t *>(smem);
+    uint16_t *pv = ph + kLdsCap;
+    uint16_t *cnt = pv + kLdsCap;   // [kBuckets][kSortWaves]
+    uint16_t *qh = cnt + kBuckets * kSortWaves;
+    uint16_t *qv = qh + kLdsCap;
+    uint32_t *sbins = reinterpret_cast<uint32_t *>(smem + kSortBytes);
     __shared__ uint32_t s_wave[kSortWaves];
     const int tid = threadIdx.x;
-    for (int i = tid; i < m; i += kSortThreads) {
-        h0[i] = key_bucket(a.keys[i]);
-        v0[i] = (uint16_t)i;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int nbins = a.guess.nbins;
+    const bool lds_bins = nbins <= kBinCap;
+    if (lds_bins)
+        for (int b = tid; b < nbins; b += kSortThreads) sbins[b] = 0;
+    for (int i = tid; i < m; i += kSortThreads) ph[i] = key_bucket(a.keys[i], kHashBits);
+    // wave w owns pairs [w_lo, w_hi) and row entries cnt[* x kSortWaves + w]
+    const int per_wave = (m + kSortWaves - 1) / kSortWaves;
+    const int w_lo = min(wave * per_wave, m);
+    const int w_hi = min(w_lo + per_wave, m);
+    const uint64_t lt_mask = (1ull << lane) - 1ull;
+    for (int d = lane; d < kBuckets; d += 64) cnt[d * kSortWaves + wave] = 0;
+    __syncthreads();
+    // count: per 64-pair step, the lowest lane of each hash adds its peers
+    for (int base = w_lo; base < w_hi; base += 64) {
+        const int i = base + lane;
+        const bool valid = i < w_hi;
+        const uint32_t h = valid ? ph[i] : 0u;
+        const uint64_t peers = hash_peers(h, valid);
+        if (valid && (peers & lt_mask) == 0)
+            cnt[h * kSortWaves + wave] = (uint16_t)(cnt[h * kSortWaves + wave] + __popcll(peers));
     }
     __syncthreads();
-    const int ipt = (m + kSortThreads - 1) / kSortThreads;
-    const int my_lo = min(tid * ipt, m);
-    const int my_hi = min(my_lo + ipt, m);
-    uint16_t *src_h = h0, *dst_h = h1;
-    uint16_t *src_v = v0, *dst_v = v1;
-    for (int p = 0; p < kHashPasses; ++p) {
-        const int shift = p * kDigitBits;
-        uint32_t c[kDigits];
-#pragma unroll
-        for (int d = 0; d < kDigits; ++d) c[d] = 0;
-        for (int i = my_lo; i < my_hi; ++i) {
-            const int d = (src_h[i] >> shift) & (kDigits - 1);
-#pragma unroll
-            for (int dd = 0; dd < kDigits; ++dd) c[dd] += (d == dd);
-        }
-#pragma unroll
-        for (int d = 0; d < kDigits; ++d) cnt[d * kSortThreads + tid] = (uint16_t)c[d];
-        __syncthreads();
-        // exclusive scan of the digit-major table; thread t scans entries
-        // [t * kDigits, (t + 1) * kDigits)
+    // exclusive scan, bucket-major: all of bucket b's pairs (waves in order)
+    // precede bucket b + 1's
+    {
         uint32_t run = 0;
-        uint32_t local[kDigits];
+        uint32_t local[kSortWaves];
 #pragma unroll
-        for (int j = 0; j < kDigits; ++j) {
-            local[j] = run;
-            run += cnt[tid * kDigits + j];
+        for (int w = 0; w < kSortWaves; ++w) {
+            local[w] = run;
+            run += cnt[tid * kSortWaves + w];
         }
         const uint32_t base = block_exclusive_scan_1024(run, s_wave);
 #pragma unroll
-        for (int j = 0; j < kDigits; ++j) cnt[tid * kDigits + j] = (uint16_t)(base + local[j]);
-        __syncthreads();
-        for (int i = my_lo; i < my_hi; ++i) {
-            const uint16_t hh = src_h[i];
-            const int d = (hh >> shift) & (kDigits - 1);
-            const uint32_t pos = cnt[d * kSortThreads + tid];
-            cnt[d * kSortThreads + tid] = (uint16_t)(pos + 1);
-            dst_h[pos] = hh;
-            dst_v[pos] = src_v[i];
+        for (int w = 0; w < kSortWaves; ++w) cnt[tid * kSortWaves + w] = (uint16_t)(base + local[w]);
+    }
+    __syncthreads();
+    // scatter in pair order: rank among lower lanes of the same hash, then
+    // the lowest lane moves the wave's cursor past the step's peers
+    for (int base = w_lo; base < w_hi; base += 64) {
+        const int i = base + lane;
+        const bool valid = i < w_hi;
+        const uint32_t h = valid ? ph[i] : 0u;
+        const uint64_t peers = hash_peers(h, valid);
+        uint32_t cur = 0;
+        if (valid) cur = cnt[h * kSortWaves + wave];
+        if (valid) {
+            const uint32_t pos = cur + (uint32_t)__popcll(peers & lt_mask);
+            qh[pos] = (uint16_t)h;
+            qv[pos] = (uint16_t)i;
+            if ((peers & lt_mask) == 0) cnt[h * kSortWaves + wave] = (uint16_t)(cur + __popcll(peers));
+        }
+    }
+    __syncthreads();
+    const uint16_t *src_h = qh;
+    const uint16_t *src_v = qv;
+    // keys and energies in sorted order, staged in the free region (P pair +
+    // counts) when they fit; else read from the pair arrays
+    uint64_t *sk = reinterpret_cast<uint64_t *>(smem);
+    double *sy = reinterpret_cast<double *>(sk + m);
+    const bool staged = m <= kStage;
+    if (staged) {
+        for (int i = tid; i < m; i += kSortThreads) {
+            const uint16_t v = src_v[i];
+            sk[i] = a.keys[v];
+            sy[i] = a.y[v];
         }
         __syncthreads();
-        uint16_t *th = src_h; src_h = dst_h; dst_h = th;
-        uint16_t *tv = src_v; src_v = dst_v; dst_v = tv;
     }
+    auto key_at = [&](int i) -> uint64_t { return staged ? sk[i] : a.keys[src_v[i]]; };
+    auto y_at = [&](int i) -> double { return staged ? sy[i] : a.y[src_v[i]]; };
     // reduce-by-key left to right (Histogram.cu:153), bin (:158): the pair at
     // sorted position i starts its key's sum if no earlier pair of its bucket
     // has the same key; the sum then takes the bucket's pairs of that key in
-    // order (ascending pair index: the sort is stable).  A thread's positions
-    // are handled together, stage by stage, so the global loads of its pairs
-    // (keys, energies, bin edges) overlap instead of queueing behind each other.
-    constexpr int kR = (kLdsCap + kSortThreads - 1) / kSortThreads;
-    uint64_t kk[kR];
-    double s[kR];
-    bool lead[kR];
-#pragma unroll
-    for (int r = 0; r < kR; ++r) {
-        const int i = tid + r * kSortThreads;
-        lead[r] = false;
-        if (i < m) {
-            const int pi = src_v[i];
-            kk[r] = a.keys[pi];
-            s[r] = a.y[pi];
-        }
-    }
-#pragma unroll
-    for (int r = 0; r < kR; ++r) {
-        const int i = tid + r * kSortThreads;
-        if (i >= m) continue;
+    // order (ascending pair index: the sort is stable).  Staged scans read
+    // kChunk pairs at a time, so only the FP64 adds stay serial.
+    constexpr int kChunk = 8;
+#pragma unroll 1
+    for (int i = tid; i < m; i += kSortThreads) {
         const uint16_t hb = src_h[i];
+        const uint64_t kk = key_at(i);
         bool first = true;
-        for (int j = i - 1; j >= 0 && src_h[j] == hb; --j) {
-            if (a.keys[src_v[j]] == kk[r]) {
-                first = false;
-                break;
+        if (staged) {
+            bool live = true;
+#pragma unroll 1
+            for (int j = i - 1; live; j -= kChunk) {
+                uint16_t hh[kChunk];
+                uint64_t kv[kChunk];
+#pragma unroll
+                for (int u = 0; u < kChunk; ++u) {
+                    const int jj = max(j - u, 0);
+                    hh[u] = src_h[jj];
+                    kv[u] = sk[jj];
+                }
+#pragma unroll
+                for (int u = 0; u < kChunk; ++u) {
+                    live = live && (j - u >= 0) && (hh[u] == hb);
+                    if (live && kv[u] == kk) {
+                        first = false;
+                        live = false;
+                    }
+                }
+            }
+        } else {
+#pragma unroll 1
+            for (int j = i - 1; j >= 0 && src_h[j] == hb; --j) {
+                if (key_at(j) == kk) {
+                    first = false;
+                    break;
+                }
             }
         }
         if (!first) continue;
-        lead[r] = true;
-        double acc = s[r];
-        for (int j = i + 1; j < m && src_h[j] == hb; ++j) {
-            const int pj = src_v[j];
-            if (a.keys[pj] == kk[r]) acc = acc + a.y[pj];
+        double acc = y_at(i);
+        if (staged) {
+            bool live = true;
+#pragma unroll 1
+            for (int j = i + 1; live; j += kChunk) {
+                uint16_t hh[kChunk];
+                uint64_t kv[kChunk];
+                double yv[kChunk];
+#pragma unroll
+                for (int u = 0; u < kChunk; ++u) {
+                    const int jj = min(j + u, m - 1);
+                    hh[u] = src_h[jj];
+                    kv[u] = sk[jj];
+                    yv[u] = sy[jj];
+                }
+#pragma unroll
+                for (int u = 0; u < kChunk; ++u) {
+                    live = live && (j + u < m) && (hh[u] == hb);
+                    if (live && kv[u] == kk) acc = acc + yv[u];
+                }
+            }
+        } else {
+#pragma unroll 1
+            for (int j = i + 1; j < m && src_h[j] == hb; ++j)
+                if (key_at(j) == kk) acc = acc + y_at(j);
         }
-        s[r] = acc;
+        if (lds_bins) bin_sum_lds(a, sbins, acc);
+        else bin_sum(a, acc);
     }
-    // binning: guesses and edge loads for all of the thread's sums first
-    int bb[kR];
-    double lo[kR], hi[kR];
-#pragma unroll
-    for (int r = 0; r < kR; ++r) {
-        const double v = lead[r] ? s[r] : a.guess.e0;
-        float t;
-        if (a.guess.type == ST_HIST_LIN) t = (float)((v - a.guess.origin) * a.guess.inv_width);
-        else t = (__log2f((float)v) - a.guess.log2_origin) * a.guess.log2_inv_width;
-        bb[r] = (t > 0.0f) ? (int)fminf(t, (float)(a.guess.nbins - 1)) : 0;
-        lo[r] = a.edges[bb[r]];
-        hi[r] = a.edges[bb[r] + 1];
-    }
-#pragma unroll
-    for (int r = 0; r < kR; ++r) {
-        if (!lead[r]) continue;
-        const double v = s[r];
-        int b = bb[r];
-        if (!(v >= lo[r] && v < hi[r])) b = bin_fast(a.edges, a.guess, v);
-        if (b >= 0) atomicAdd(&a.bins[b], 1ull);
+    if (!lds_bins) return;
+    __syncthreads();
+    for (int b = tid; b < nbins; b += kSortThreads) {
+        const uint32_t c = sbins[b];
+        if (c) atomicAdd(&a.bins[b], (unsigned long long)c);
     }
 }
 

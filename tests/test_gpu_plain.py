@@ -141,7 +141,9 @@ def test_histogram_tests_known_answers(capacity):
                                                  (4000, 1 << 27, 100000), (0, 10, 100000),
                                                  (100000, 1 << 35, 200000), (131072, 1 << 20, None),
                                                  (20000, 3, None), (12288, 1 << 20, None), (12289, 1 << 40, None),
-                                                 (12000, 1, None), (9000, 1 << 62, 12288)])
+                                                 (12000, 1, None), (9000, 1 << 62, 12288),
+                                                 (3000, 2, None), (4096, 1 << 50, None), (4097, 4, None),
+                                                 (5120, 5, None), (5121, 3, None)])
 def test_histogram_random_pairs_match_oracle(m, keyspace, capacity):
     rng = np.random.default_rng(m + 3)
     keys = rng.integers(0, keyspace, m, dtype=np.uint64)
@@ -151,6 +153,20 @@ def test_histogram_random_pairs_match_oracle(m, keyspace, capacity):
     edges, got = hist_from_pairs("log", 1000, 1e-2, 1e3, keys, ys, capacity)
     _, sums = po.reduce_pairs(keys, ys)
     want = po.histogram_counts(edges, sums)
+    np.testing.assert_array_equal(got, want.astype(np.float64))
+
+
+@pytest.mark.parametrize("nb", [4096, 4097, 20000])
+def test_histogram_many_bins_match_oracle(nb):
+    # up to 4096 bins are counted in LDS per launch, more go to global counters
+    rng = np.random.default_rng(nb)
+    m = 3000
+    keys = rng.integers(0, 200, m).astype(np.uint64)
+    ys = rng.uniform(0.0, 2.0, m)
+    edges, got = hist_from_pairs("lin", nb, 0.0, 60.0, keys, ys)
+    _, sums = po.reduce_pairs(keys, ys)
+    want = po.histogram_counts(edges, sums)
+    assert want.sum() == len(np.unique(keys))
     np.testing.assert_array_equal(got, want.astype(np.float64))
 
 
