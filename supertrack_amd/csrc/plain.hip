@@ -299,11 +299,9 @@ __device__ __forceinline__ void bin_sum(const HistArgs &a, double s)
     if (b >= 0) atomicAdd(&a.bins[b], 1ull);
 }
 
-__device__ __forceinline__ void bin_sum_lds(const HistArgs &a, uint32_t *sbins, double s)
-{
-    const int b = bin_fast(a.edges, a.guess, s);
-    if (b >= 0) atomicAdd(&sbins[b], 1u);
-}
+// Histograms of up to kEdgeRegs * kSortThreads - 1 bins keep their edges in
+// LDS when the free region has room (each sum's bin then costs no global trip)
+constexpr int kEdgeRegs = 4;
 
 // Lanes of the wave whose valid flag is set and whose hash equals this lane's.
 __device__ __forceinline__ uint64_t hash_peers(uint32_t h, bool valid)
@@ -335,6 +333,18 @@ __device__ void lds_sort_reduce_bin(const HistArgs &a, int m, unsigned char *sme
     const bool lds_bins = nbins <= kBinCap;
     if (lds_bins)
         for (int b = tid; b < nbins; b += kSortThreads) sbins[b] = 0;
+    // edges and energies staged in the free region after the sort, if both fit
+    const bool staged = m <= kStage;
+    const bool lds_edges = staged && nbins < kEdgeRegs * kSortThreads &&
+                           (size_t)m * 16 + (size_t)(nbins + 1) * 8 <= (size_t)kStage * 16;
+    double er[kEdgeRegs];
+    if (lds_edges) {
+#pragma unroll
+        for (int r = 0; r < kEdgeRegs; ++r) {
+            const int e = tid + r * kSortThreads;
+            er[r] = e <= nbins ? a.edges[e] : 0.0;
+        }
+    }
     for (int i = tid; i < m; i += kSortThreads) ph[i] = key_bucket(a.keys[i], kHashBits);
     // wave w owns pairs [w_lo, w_hi) and row entries cnt[* x kSortWaves + w]
     const int per_wave = (m + kSortWaves - 1) / kSortWaves;
@@ -388,32 +398,46 @@ __device__ void lds_sort_reduce_bin(const HistArgs &a, int m, unsigned char *sme
     const uint16_t *src_h = qh;
     const uint16_t *src_v = qv;
     // keys and energies in sorted order, staged in the free region (P pair +
-    // counts) when they fit; else read from the pair arrays
+    // counts) when they fit, edges after them; else read from global memory
     uint64_t *sk = reinterpret_cast<uint64_t *>(smem);
     double *sy = reinterpret_cast<double *>(sk + m);
-    const bool staged = m <= kStage;
+    double *se = sy + m;
     if (staged) {
         for (int i = tid; i < m; i += kSortThreads) {
             const uint16_t v = src_v[i];
             sk[i] = a.keys[v];
             sy[i] = a.y[v];
         }
+        if (lds_edges) {
+#pragma unroll
+            for (int r = 0; r < kEdgeRegs; ++r) {
+                const int e = tid + r * kSortThreads;
+                if (e <= nbins) se[e] = er[r];
+            }
+        }
         __syncthreads();
     }
-    auto key_at = [&](int i) -> uint64_t { return staged ? sk[i] : a.keys[src_v[i]]; };
-    auto y_at = [&](int i) -> double { return staged ? sy[i] : a.y[src_v[i]]; };
+    auto add_bin = [&](double v) {
+        const int b = lds_edges ? bin_fast(se, a.guess, v) : bin_fast(a.edges, a.guess, v);
+        if (b >= 0) {
+            if (lds_bins) atomicAdd(&sbins[b], 1u);
+            else atomicAdd(&a.bins[b], 1ull);
+        }
+    };
     // reduce-by-key left to right (Histogram.cu:153), bin (:158): the pair at
     // sorted position i starts its key's sum if no earlier pair of its bucket
     // has the same key; the sum then takes the bucket's pairs of that key in
-    // order (ascending pair index: the sort is stable).  Staged scans read
-    // kChunk pairs at a time, so only the FP64 adds stay serial.
+    // order (ascending pair index: the sort is stable).  The staged loop reads
+    // LDS only (kChunk pairs at a time, so only the FP64 adds stay serial);
+    // the other reads global memory only: a loop that mixes the two gets
+    // generic loads, whose waits also cover the global ones.
     constexpr int kChunk = 8;
+    if (staged) {
 #pragma unroll 1
-    for (int i = tid; i < m; i += kSortThreads) {
-        const uint16_t hb = src_h[i];
-        const uint64_t kk = key_at(i);
-        bool first = true;
-        if (staged) {
+        for (int i = tid; i < m; i += kSortThreads) {
+            const uint16_t hb = src_h[i];
+            const uint64_t kk = sk[i];
+            bool first = true;
             bool live = true;
 #pragma unroll 1
             for (int j = i - 1; live; j -= kChunk) {
@@ -434,19 +458,9 @@ __device__ void lds_sort_reduce_bin(const HistArgs &a, int m, unsigned char *sme
                     }
                 }
             }
-        } else {
-#pragma unroll 1
-            for (int j = i - 1; j >= 0 && src_h[j] == hb; --j) {
-                if (key_at(j) == kk) {
-                    first = false;
-                    break;
-                }
-            }
-        }
-        if (!first) continue;
-        double acc = y_at(i);
-        if (staged) {
-            bool live = true;
+            if (!first) continue;
+            double acc = sy[i];
+            live = true;
 #pragma unroll 1
             for (int j = i + 1; live; j += kChunk) {
                 uint16_t hh[kChunk];
@@ -465,13 +479,28 @@ __device__ void lds_sort_reduce_bin(const HistArgs &a, int m, unsigned char *sme
                     if (live && kv[u] == kk) acc = acc + yv[u];
                 }
             }
-        } else {
+            add_bin(acc);
+        }
+    } else {
+#pragma unroll 1
+        for (int i = tid; i < m; i += kSortThreads) {
+            const uint16_t hb = src_h[i];
+            const uint64_t kk = a.keys[src_v[i]];
+            bool first = true;
+#pragma unroll 1
+            for (int j = i - 1; j >= 0 && src_h[j] == hb; --j) {
+                if (a.keys[src_v[j]] == kk) {
+                    first = false;
+                    break;
+                }
+            }
+            if (!first) continue;
+            double acc = a.y[src_v[i]];
 #pragma unroll 1
             for (int j = i + 1; j < m && src_h[j] == hb; ++j)
-                if (key_at(j) == kk) acc = acc + y_at(j);
+                if (a.keys[src_v[j]] == kk) acc = acc + a.y[src_v[j]];
+            add_bin(acc);
         }
-        if (lds_bins) bin_sum_lds(a, sbins, acc);
-        else bin_sum(a, acc);
     }
     if (!lds_bins) return;
     __syncthreads();

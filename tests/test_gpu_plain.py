@@ -156,9 +156,10 @@ def test_histogram_random_pairs_match_oracle(m, keyspace, capacity):
     np.testing.assert_array_equal(got, want.astype(np.float64))
 
 
-@pytest.mark.parametrize("nb", [4096, 4097, 20000])
+@pytest.mark.parametrize("nb", [2000, 4095, 4096, 4097, 20000])
 def test_histogram_many_bins_match_oracle(nb):
-    # up to 4096 bins are counted in LDS per launch, more go to global counters
+    # up to 4096 bins are counted in LDS per launch, more go to global counters;
+    # edges of up to 4095 bins are staged in LDS when they fit beside the pairs
     rng = np.random.default_rng(nb)
     m = 3000
     keys = rng.integers(0, 200, m).astype(np.uint64)
