@@ -92,13 +92,31 @@ class Events:
 
 
 def build_library(w, n_tracks, precision):
+    """Device library of the workload's first n_tracks tracks.  Mixed-geometry
+    workloads are generated, laid out and uploaded one same-geometry run at a
+    time (host arrays: None); the others return their host arrays too."""
     import supertrack_amd as st
     from supertrack_amd import workloads as W
+    geoms = W.geometries(w)
     t0 = time.time()
+    if len(geoms) > 1:
+        nt, npts, chunks = W.library_chunks(w, n_tracks)
+        log(f"library: {nt} tracks, {npts} points, streamed by geometry run")
+
+        def logged(it):
+            done = 0
+            for c in it:
+                yield c
+                done += len(c[5])
+                log(f"library: {done}/{nt} tracks appended ({time.time() - t0:.1f}s)")
+
+        lib = st.TrackLibrary.streamed(nt, npts, logged(chunks), geoms, precision)
+        log(f"library: {n_tracks} tracks, {npts} points; generate+layout+upload by geometry run "
+            f"{time.time() - t0:.1f}s; {lib.info()}")
+        return lib, None
     x, y, z, e, off, gt = W.make_library(w, n_tracks=n_tracks)
     t1 = time.time()
-    geoms = W.geometries(w)
-    lib = st.TrackLibrary(x, y, z, e, off, geoms, gt if len(geoms) > 1 else None, precision)
+    lib = st.TrackLibrary(x, y, z, e, off, geoms, None, precision)
     t2 = time.time()
     log(f"library: {n_tracks} tracks, {int(off[-1])} points; generate {t1 - t0:.1f}s, "
         f"layout+upload {t2 - t1:.1f}s; {lib.info()}")

@@ -168,6 +168,20 @@ int st_library_create(const double *x, const double *y, const double *z, const d
                       const int64_t *offsets, int64_t n_tracks,
                       const st_geometry *geoms, int32_t n_geoms, const int32_t *geom_of_track,
                       int32_t precision, st_library **out);
+/* Streamed construction of the same library, for libraries whose host copy
+ * would not fit in memory at once: st_library_begin sizes the device arrays
+ * for n_tracks / n_points in total; st_library_append lays out and uploads
+ * the next tracks in library order (host arrays as for st_library_create,
+ * indexed by this call's offsets; geom_of_track may be NULL); after the last
+ * append, st_library_finish publishes the track table.  st_batch_create
+ * accepts only a finished library.  st_library_create is begin, appends of
+ * bounded size, finish.  Appending more tracks or points than begin sized, or
+ * finishing with fewer, is ST_EINVAL. */
+int st_library_begin(int64_t n_tracks, int64_t n_points, const st_geometry *geoms, int32_t n_geoms,
+                     int32_t precision, st_library **out);
+int st_library_append(st_library *lib, const double *x, const double *y, const double *z, const double *edep,
+                      const int64_t *offsets, int64_t n_tracks, const int32_t *geom_of_track);
+int st_library_finish(st_library *lib);
 int st_library_info_get(const st_library *lib, st_library_info *info);
 void st_library_destroy(st_library *lib);
 

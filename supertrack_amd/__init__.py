@@ -317,6 +317,32 @@ class TrackLibrary:
              int(precision), C.byref(self.h))
         self.geometries = list(geometries)
 
+    @classmethod
+    def streamed(cls, n_tracks: int, n_points: int, chunks: Iterable, geometries: Sequence[SphericalGeometry],
+                 precision: int = ST_F64) -> "TrackLibrary":
+        """Builds the library from an iterable of (x, y, z, e, offsets, geom_of_track)
+        chunks in library order, holding one chunk on the host at a time;
+        n_tracks and n_points are the totals over all chunks."""
+        self = cls.__new__(cls)
+        self.h = C.c_void_p()
+        self.geometries = list(geometries)
+        garr = (_capi.Geometry * len(geometries))(*[g.c for g in geometries])
+        call("st_library_begin", C.c_int64(n_tracks), C.c_int64(n_points), garr, len(geometries), int(precision),
+             C.byref(self.h))
+        for x, y, z, e, offsets, gt in chunks:
+            xs, px = _dptr(x, np.float64)
+            ys, py = _dptr(y, np.float64)
+            zs, pz = _dptr(z, np.float64)
+            es, pe = _dptr(e, np.float64)
+            offs, po = _dptr(offsets, np.int64)
+            if gt is not None:
+                gts, pg = _dptr(gt, np.int32)
+            else:
+                gts, pg = None, None
+            call("st_library_append", self.h, px, py, pz, pe, po, C.c_int64(len(offs) - 1), pg)
+        call("st_library_finish", self.h)
+        return self
+
     def info(self) -> dict:
         i = _capi.LibraryInfo()
         call("st_library_info_get", self.h, C.byref(i))
@@ -408,6 +434,17 @@ class BatchRun:
             pass
 
 
+def synth_counts(spec: dict, n_tracks: int, t0: int = 0) -> np.ndarray:
+    """Point counts of tracks [t0, t0 + n_tracks) of a synthetic spec, without generating them."""
+    s = _capi.SynthSpec()
+    for k, v in spec.items():
+        setattr(s, k, v)
+    counts = np.zeros(n_tracks, np.int64)
+    call("st_synth_counts", C.byref(s), C.c_int64(t0), C.c_int64(t0 + n_tracks),
+         counts.ctypes.data_as(_capi.c_int64_p))
+    return counts
+
+
 def synth_library(spec: dict, n_tracks: int, t0: int = 0):
     """Generates tracks [t0, t0 + n_tracks) of a synthetic spec.
 
@@ -416,9 +453,7 @@ def synth_library(spec: dict, n_tracks: int, t0: int = 0):
     s = _capi.SynthSpec()
     for k, v in spec.items():
         setattr(s, k, v)
-    counts = np.zeros(n_tracks, np.int64)
-    call("st_synth_counts", C.byref(s), C.c_int64(t0), C.c_int64(t0 + n_tracks),
-         counts.ctypes.data_as(_capi.c_int64_p))
+    counts = synth_counts(spec, n_tracks, t0)
     offsets = np.zeros(n_tracks + 1, np.int64)
     np.cumsum(counts, out=offsets[1:])
     n = int(offsets[-1])

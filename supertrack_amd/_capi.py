@@ -100,6 +100,11 @@ PROTOTYPES = {
     "st_library_create": (C.c_int, [c_double_p, c_double_p, c_double_p, c_double_p, c_int64_p, C.c_int64,
                                     C.POINTER(Geometry), C.c_int32, c_int32_p, C.c_int32,
                                     C.POINTER(C.c_void_p)]),
+    "st_library_begin": (C.c_int, [C.c_int64, C.c_int64, C.POINTER(Geometry), C.c_int32, C.c_int32,
+                                   C.POINTER(C.c_void_p)]),
+    "st_library_append": (C.c_int, [C.c_void_p, c_double_p, c_double_p, c_double_p, c_double_p, c_int64_p,
+                                    C.c_int64, c_int32_p]),
+    "st_library_finish": (C.c_int, [C.c_void_p]),
     "st_library_info_get": (C.c_int, [C.c_void_p, C.POINTER(LibraryInfo)]),
     "st_library_destroy": (None, [C.c_void_p]),
     "st_batch_create": (C.c_int, [C.c_void_p, C.POINTER(BatchOpts), C.POINTER(C.c_void_p), C.c_int32,

@@ -719,6 +719,29 @@ size_t lds_bytes(int mode, int slots, int hist_words)
 }
 
 template <typename T>
+int dev_alloc(T **dst, size_t n, int64_t *bytes)
+{
+    *dst = nullptr;
+    if (n == 0) return ST_OK;
+    hipError_t e = hipMalloc((void **)dst, n * sizeof(T));
+    if (e != hipSuccess) {
+        *dst = nullptr;
+        return set_error(ST_ENOMEM, "st_library_begin: hipMalloc(%zu B) failed: %s", n * sizeof(T),
+                         hipGetErrorString(e));
+    }
+    *bytes += (int64_t)(n * sizeof(T));
+    return ST_OK;
+}
+
+template <typename T>
+int dev_put(T *dst, const T *src, size_t n)
+{
+    if (n == 0) return ST_OK;
+    ST_HIP_TRY(hipMemcpy(dst, src, n * sizeof(T), hipMemcpyHostToDevice));
+    return ST_OK;
+}
+
+template <typename T>
 int dev_upload(T **dst, const T *src, size_t n, int64_t *bytes)
 {
     *dst = nullptr;
@@ -787,6 +810,10 @@ struct st_library {
     BlockF32 *blocks = nullptr;
     TrackDesc *tracks_dev = nullptr;
     GeomDev *geoms_dev = nullptr;
+    // streamed construction: begin sizes the device arrays, appends fill them
+    // in track order, finish publishes the track table
+    int64_t tracks_done = 0, points_done = 0, blocks_cap = 0;
+    bool finished = false;
 };
 
 struct st_batch {
@@ -942,52 +969,92 @@ int run_exact(st_batch *b)
 
 extern "C" {
 
-int st_library_create(const double *x, const double *y, const double *z, const double *edep,
-                      const int64_t *offsets, int64_t n_tracks, const st_geometry *geoms, int32_t n_geoms,
-                      const int32_t *geom_of_track, int32_t precision, st_library **out)
+int st_library_begin(int64_t n_tracks, int64_t n_points, const st_geometry *geoms, int32_t n_geoms,
+                     int32_t precision, st_library **out)
 {
-    if (!out) return set_error(ST_EINVAL, "st_library_create: null output");
+    if (!out) return set_error(ST_EINVAL, "st_library_begin: null output");
     *out = nullptr;
-    if (n_tracks < 0 || !offsets || !geoms || n_geoms < 1 || n_geoms > kMaxGeoms)
-        return set_error(ST_EINVAL, "st_library_create: bad arguments (1 <= n_geoms <= %d)", kMaxGeoms);
+    if (n_tracks < 0 || n_points < 0 || !geoms || n_geoms < 1 || n_geoms > kMaxGeoms)
+        return set_error(ST_EINVAL, "st_library_begin: bad arguments (1 <= n_geoms <= %d)", kMaxGeoms);
     if (precision != ST_F64 && precision != ST_F32)
-        return set_error(ST_EINVAL, "st_library_create: precision must be ST_F64 or ST_F32");
-    const int64_t n_points = offsets[n_tracks] - offsets[0];
-    if (n_points > 0 && (!x || !y || !z || !edep)) return set_error(ST_EINVAL, "st_library_create: null arrays");
-    for (int64_t t = 0; t < n_tracks; ++t) {
-        const int64_t n = offsets[t + 1] - offsets[t];
-        if (n < 0 || n > 0x7FFFFFFF) return set_error(ST_EINVAL, "st_library_create: bad offsets at track %lld",
-                                                      (long long)t);
-        if (geom_of_track && (geom_of_track[t] < 0 || geom_of_track[t] >= n_geoms))
-            return set_error(ST_EINVAL, "st_library_create: geometry index out of range at track %lld",
-                             (long long)t);
-    }
+        return set_error(ST_EINVAL, "st_library_begin: precision must be ST_F64 or ST_F32");
     for (int i = 0; i < n_geoms; ++i)
         if (!(geoms[i].diameter > 0) || geoms[i].n_linear < 1)
-            return set_error(ST_EINVAL, "st_library_create: invalid geometry %d", i);
-
+            return set_error(ST_EINVAL, "st_library_begin: invalid geometry %d", i);
     st_library *L = new (std::nothrow) st_library;
-    if (!L) return set_error(ST_ENOMEM, "st_library_create: out of host memory");
+    if (!L) return set_error(ST_ENOMEM, "st_library_begin: out of host memory");
     L->n_tracks = n_tracks;
     L->n_points = n_points;
     L->precision = precision;
     L->geoms.assign(geoms, geoms + n_geoms);
     L->tracks.resize(n_tracks);
+    // ceil(n / kBlk) <= n / kBlk + 1 per track
+    L->blocks_cap = n_points / kBlk + n_tracks + 1;
+    int rc = ST_OK;
+    int64_t bytes = 0;
+    if (!rc) rc = dev_alloc(&L->x64, n_points, &bytes);
+    if (!rc) rc = dev_alloc(&L->y64, n_points, &bytes);
+    if (!rc) rc = dev_alloc(&L->z64, n_points, &bytes);
+    if (!rc) rc = dev_alloc(&L->e64, n_points, &bytes);
+    if (!rc) rc = dev_alloc(&L->zfirst, L->blocks_cap, &bytes);
+    if (!rc && precision == ST_F32) {
+        rc = dev_alloc(&L->p32, n_points, &bytes);
+        if (!rc) rc = dev_alloc(&L->blocks, L->blocks_cap, &bytes);
+    }
+    L->device_bytes = bytes;
+    if (rc) {
+        st_library_destroy(L);
+        return rc;
+    }
+    *out = L;
+    return ST_OK;
+}
 
-    // block layout
+int st_library_append(st_library *L, const double *x, const double *y, const double *z, const double *edep,
+                      const int64_t *offsets, int64_t n_tracks, const int32_t *geom_of_track)
+{
+    if (!L || !offsets || n_tracks < 0) return set_error(ST_EINVAL, "st_library_append: bad arguments");
+    if (L->finished) return set_error(ST_EINVAL, "st_library_append: library already finished");
+    const int n_geoms = (int)L->geoms.size();
+    const int64_t n_points = offsets[n_tracks] - offsets[0];
+    if (n_points > 0 && (!x || !y || !z || !edep)) return set_error(ST_EINVAL, "st_library_append: null arrays");
+    for (int64_t t = 0; t < n_tracks; ++t) {
+        const int64_t n = offsets[t + 1] - offsets[t];
+        if (n < 0 || n > 0x7FFFFFFF)
+            return set_error(ST_EINVAL, "st_library_append: bad offsets at track %lld", (long long)t);
+        if (geom_of_track && (geom_of_track[t] < 0 || geom_of_track[t] >= n_geoms))
+            return set_error(ST_EINVAL, "st_library_append: geometry index out of range at track %lld",
+                             (long long)t);
+    }
+    if (n_tracks > L->n_tracks - L->tracks_done || n_points > L->n_points - L->points_done)
+        return set_error(ST_EINVAL,
+                         "st_library_append: %lld tracks / %lld points exceed what st_library_begin sized "
+                         "(%lld / %lld left)",
+                         (long long)n_tracks, (long long)n_points, (long long)(L->n_tracks - L->tracks_done),
+                         (long long)(L->n_points - L->points_done));
+    const int precision = L->precision;
+    const int64_t pbase = L->points_done;   // this chunk's first point in the library
+    int64_t bbase = 0;                       // this chunk's first block in the library
+    if (L->tracks_done > 0) {
+        const TrackDesc &last = L->tracks[L->tracks_done - 1];
+        bbase = last.block0 + last.nblocks;
+    }
+    TrackDesc *tds = L->tracks.data() + L->tracks_done;
     int64_t nblocks = 0;
     for (int64_t t = 0; t < n_tracks; ++t) {
-        TrackDesc &td = L->tracks[t];
-        td.offset = offsets[t] - offsets[0];
+        TrackDesc &td = tds[t];
+        td.offset = pbase + (offsets[t] - offsets[0]);
         td.n = (int32_t)(offsets[t + 1] - offsets[t]);
         td.nblocks = (td.n + kBlk - 1) / kBlk;
-        td.block0 = nblocks;
+        td.block0 = bbase + nblocks;
         td.geom = geom_of_track ? geom_of_track[t] : 0;
         td.flags = 0;
         nblocks += td.nblocks;
     }
-    L->n_blocks = nblocks;
+    if (bbase + nblocks > L->blocks_cap)
+        return set_error(ST_EINVAL, "st_library_append: block table overflow");
 
+    // host staging for this chunk, indexed from the chunk's first point/block
     std::vector<double> sx(n_points), sy(n_points), sz(n_points), se(n_points);
     std::vector<double> zfirst(std::max<int64_t>(nblocks, 1));
     std::vector<float4> p32;
@@ -999,8 +1066,9 @@ int st_library_create(const double *x, const double *y, const double *z, const d
     const double *X = x, *Y = y, *Z = z, *E = edep;
     const int64_t base = offsets[0];
     parallel_for(n_tracks, [&](int64_t t) {
-        TrackDesc &td = L->tracks[t];
-        const int64_t o = td.offset, n = td.n;
+        TrackDesc &td = tds[t];
+        const int64_t o = td.offset - pbase, n = td.n;
+        const int64_t b0 = td.block0 - bbase;
         std::vector<int64_t> idx(n);
         std::iota(idx.begin(), idx.end(), 0);
         const double *zt = Z + base + o;
@@ -1028,7 +1096,7 @@ int st_library_create(const double *x, const double *y, const double *z, const d
         td.ymin = ymin;
         td.ymax = ymax;
         td.zmax = n ? sz[o + n - 1] : 0.0;
-        for (int b = 0; b < td.nblocks; ++b) zfirst[td.block0 + b] = sz[o + (int64_t)b * kBlk];
+        for (int b = 0; b < td.nblocks; ++b) zfirst[b0 + b] = sz[o + (int64_t)b * kBlk];
         const st_geometry &g = L->geoms[td.geom];
         const double denom = (2. / 3.) * g.diameter;
         // fixed-point headroom: sum of |y| over the track
@@ -1074,7 +1142,7 @@ int st_library_create(const double *x, const double *y, const double *z, const d
                 // dl: the FP32 rounding of the offsets plus, with a wide margin,
                 // the FP64 rounding of each add and product on the way to the
                 // sphere test (|shift| <= H' on each axis)
-                BlockF32 &bk = blocks[td.block0 + b];
+                BlockF32 &bk = blocks[b0 + b];
                 const double invD = 1.0 / g.diameter;
                 bk.oqx = (org[0] - g.gso) * invD;
                 bk.oqy = (org[1] - g.gso) * invD;
@@ -1093,13 +1161,41 @@ int st_library_create(const double *x, const double *y, const double *z, const d
         }
     });
 
-    for (int64_t t = 0; t < n_tracks; ++t) {
+    int rc = ST_OK;
+    if (!rc) rc = dev_put(L->x64 + pbase, sx.data(), n_points);
+    if (!rc) rc = dev_put(L->y64 + pbase, sy.data(), n_points);
+    if (!rc) rc = dev_put(L->z64 + pbase, sz.data(), n_points);
+    if (!rc) rc = dev_put(L->e64 + pbase, se.data(), n_points);
+    if (!rc) rc = dev_put(L->zfirst + bbase, zfirst.data(), nblocks);
+    if (!rc && precision == ST_F32) {
+        rc = dev_put(L->p32 + pbase, p32.data(), n_points);
+        if (!rc) rc = dev_put(L->blocks + bbase, blocks.data(), nblocks);
+    }
+    if (rc) return rc;
+    L->tracks_done += n_tracks;
+    L->points_done += n_points;
+    return ST_OK;
+}
+
+int st_library_finish(st_library *L)
+{
+    if (!L) return set_error(ST_EINVAL, "st_library_finish: null library");
+    if (L->finished) return ST_OK;
+    if (L->tracks_done != L->n_tracks || L->points_done != L->n_points)
+        return set_error(ST_EINVAL, "st_library_finish: %lld of %lld tracks and %lld of %lld points appended",
+                         (long long)L->tracks_done, (long long)L->n_tracks, (long long)L->points_done,
+                         (long long)L->n_points);
+    L->n_blocks = L->n_tracks ? L->tracks.back().block0 + L->tracks.back().nblocks : 0;
+    L->exact.clear();
+    L->fast_order.clear();
+    for (int64_t t = 0; t < L->n_tracks; ++t) {
         if (L->tracks[t].flags & kFlagExact) L->exact.push_back((int32_t)t);
         else L->fast_order.push_back((int32_t)t);
     }
     std::stable_sort(L->fast_order.begin(), L->fast_order.end(),
                      [&](int32_t a2, int32_t b2) { return L->tracks[a2].n > L->tracks[b2].n; });
 
+    const int n_geoms = (int)L->geoms.size();
     std::vector<GeomDev> gd(n_geoms);
     for (int i = 0; i < n_geoms; ++i) {
         const st_geometry &g = L->geoms[i];
@@ -1125,21 +1221,39 @@ int st_library_create(const double *x, const double *y, const double *z, const d
         gd[i].N = (unsigned long long)g.n_linear;
         gd[i].N2 = gd[i].N * gd[i].N;
     }
-
     int rc = ST_OK;
     int64_t bytes = 0;
-    if (!rc) rc = dev_upload(&L->x64, sx.data(), n_points, &bytes);
-    if (!rc) rc = dev_upload(&L->y64, sy.data(), n_points, &bytes);
-    if (!rc) rc = dev_upload(&L->z64, sz.data(), n_points, &bytes);
-    if (!rc) rc = dev_upload(&L->e64, se.data(), n_points, &bytes);
-    if (!rc) rc = dev_upload(&L->zfirst, zfirst.data(), zfirst.size(), &bytes);
-    if (!rc && precision == ST_F32) {
-        rc = dev_upload(&L->p32, p32.data(), n_points, &bytes);
-        if (!rc) rc = dev_upload(&L->blocks, blocks.data(), blocks.size(), &bytes);
-    }
     if (!rc) rc = dev_upload(&L->tracks_dev, L->tracks.data(), L->tracks.size(), &bytes);
     if (!rc) rc = dev_upload(&L->geoms_dev, gd.data(), gd.size(), &bytes);
-    L->device_bytes = bytes;
+    L->device_bytes += bytes;
+    if (rc) return rc;
+    L->finished = true;
+    return ST_OK;
+}
+
+int st_library_create(const double *x, const double *y, const double *z, const double *edep,
+                      const int64_t *offsets, int64_t n_tracks, const st_geometry *geoms, int32_t n_geoms,
+                      const int32_t *geom_of_track, int32_t precision, st_library **out)
+{
+    if (!out) return set_error(ST_EINVAL, "st_library_create: null output");
+    *out = nullptr;
+    if (n_tracks < 0 || !offsets) return set_error(ST_EINVAL, "st_library_create: bad arguments");
+    const int64_t n_points = offsets[n_tracks] - offsets[0];
+    if (n_points < 0) return set_error(ST_EINVAL, "st_library_create: bad offsets");
+    st_library *L = nullptr;
+    int rc = st_library_begin(n_tracks, n_points, geoms, n_geoms, precision, &L);
+    if (rc) return rc;
+    // append runs of tracks of at most kStagePoints points (or one larger
+    // track), so host staging stays bounded
+    constexpr int64_t kStagePoints = int64_t(1) << 25;
+    for (int64_t t0 = 0; t0 < n_tracks && !rc;) {
+        int64_t t1 = t0 + 1;
+        while (t1 < n_tracks && offsets[t1 + 1] - offsets[t0] <= kStagePoints) ++t1;
+        rc = st_library_append(L, x, y, z, edep, offsets + t0, t1 - t0,
+                               geom_of_track ? geom_of_track + t0 : nullptr);
+        t0 = t1;
+    }
+    if (!rc) rc = st_library_finish(L);
     if (rc) {
         st_library_destroy(L);
         return rc;
@@ -1179,6 +1293,8 @@ int st_batch_create(st_library *lib, const st_batch_opts *opts, st_hist *const *
     if (!lib || !opts || (n_hists > 0 && !hists) || n_hists < 1 || n_hists > kMaxHists)
         return set_error(ST_EINVAL, "st_batch_create: need a library and 1..%d histograms", kMaxHists);
     if (opts->n_oversamples < 0) return set_error(ST_EINVAL, "st_batch_create: n_oversamples < 0");
+    if (!lib->finished)
+        return set_error(ST_EINVAL, "st_batch_create: library not finished (st_library_finish)");
     for (int i = 0; i < n_hists; ++i)
         if (!hists[i]) return set_error(ST_EINVAL, "st_batch_create: null histogram %d", i);
     st_batch *b = new (std::nothrow) st_batch;

@@ -17,7 +17,7 @@ import copy
 
 import numpy as np
 
-from . import ST_F32, ST_F64, SphericalGeometry, synth_library
+from . import ST_F32, ST_F64, SphericalGeometry, synth_counts, synth_library
 
 SEED = 1234
 
@@ -79,6 +79,45 @@ def geom_of_track(w: dict, n_tracks: int, t0: int = 0) -> np.ndarray:
         return np.zeros(n_tracks, np.int32)
     t = np.arange(t0, t0 + n_tracks, dtype=np.int64)
     return ((t // w["chunk"]) % ng).astype(np.int32)
+
+
+def _geometry_runs(w: dict, n_tracks: int, max_tracks: int):
+    """(first track, count, geometry) of consecutive same-geometry runs of at most max_tracks tracks."""
+    g_of_t = geom_of_track(w, n_tracks)
+    runs = []
+    t = 0
+    while t < n_tracks:
+        u = t + 1
+        while u < n_tracks and u - t < max_tracks and g_of_t[u] == g_of_t[t]:
+            u += 1
+        runs.append((t, u - t, int(g_of_t[t])))
+        t = u
+    return runs
+
+
+def _spec_for(w: dict, g: SphericalGeometry) -> dict:
+    spec = dict(w["synth"])
+    spec["entry_half_length"] = g.half_length
+    spec["voxel_half_length"] = g.half_length
+    return spec
+
+
+def library_chunks(w: dict, n_tracks: int = None, max_tracks: int = 1024):
+    """The library of make_library(w, n_tracks) as (n_tracks, n_points, chunks): chunks
+    yields (x, y, z, edep, offsets, geom_of_track) per run of at most max_tracks
+    consecutive same-geometry tracks, generated when it is consumed, so only one
+    run is held on the host at a time."""
+    n_tracks = w["n_tracks"] if n_tracks is None else n_tracks
+    geoms = geometries(w)
+    runs = _geometry_runs(w, n_tracks, max_tracks)
+    n_points = int(sum(int(synth_counts(_spec_for(w, geoms[g]), c, t0).sum()) for t0, c, g in runs))
+
+    def gen():
+        for t0, c, g in runs:
+            x, y, z, e, off = synth_library(_spec_for(w, geoms[g]), c, t0)
+            yield x, y, z, e, off, np.full(c, g, np.int32)
+
+    return n_tracks, n_points, gen()
 
 
 def make_library(w: dict, n_tracks: int = None, t0: int = 0, stride: int = 1):

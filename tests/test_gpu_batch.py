@@ -304,6 +304,83 @@ def test_batch_ragged_library_with_empty_and_tiny_tracks(precision):
     check_bins(counts, edges, oracle, rtol)
 
 
+def _small_mixed_workload():
+    w = W.workload("cfg5")
+    w["chunk"] = 2
+    w["synth"]["n_max"] = 30000
+    return w
+
+
+@pytest.mark.parametrize("precision", [st.ST_F64, st.ST_F32])
+def test_batch_streamed_library_matches_create(precision):
+    # begin / append (runs of at most 3 same-geometry tracks) / finish lays out
+    # the same library as st_library_create over the whole arrays
+    w = _small_mixed_workload()
+    w["chunk"] = 4   # runs of 4 tracks per geometry, appended as 3 + 1
+    geoms = W.geometries(w)
+    nt, npts, chunks = W.library_chunks(w, 12, max_tracks=3)
+    chunks = list(chunks)
+    assert [len(c[5]) for c in chunks] == [3, 1, 3, 1, 3, 1]
+    x, y, z, e = (np.concatenate([c[k] for c in chunks]) for k in range(4))
+    counts = np.concatenate([np.diff(c[4]) for c in chunks])
+    off = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    gt = np.concatenate([c[5] for c in chunks])
+    assert nt == 12 and npts == len(x) == off[-1]
+    hists = [("lin", 3000, 0.01, 300.0), ("log", 1000, 1e-2, 1e3)]
+    res = []
+    for streamed in (False, True):
+        if streamed:
+            lib = st.TrackLibrary.streamed(nt, npts, iter(chunks), geoms, precision)
+        else:
+            lib = st.TrackLibrary(x, y, z, e, off, geoms, gt, precision)
+        hs = [st.Histogram(nb, lo, hi, kind) for kind, nb, lo, hi in hists]
+        b = st.BatchRun(lib, hs, 4, 5, True, True)
+        b.run()
+        b.sync()
+        info = lib.info()
+        res.append(([h.fetch() for h in hs], b.stats(), info["fast_tracks"], info["n_points"]))
+        b.free()
+        for h in hs:
+            h.free()
+        lib.free()
+    (c0, s0, f0, n0), (c1, s1, f1, n1) = res
+    for a, b2 in zip(c0, c1):
+        np.testing.assert_array_equal(a, b2)
+    assert s0 == s1 and f0 == f1 and n0 == n1 == npts
+    assert sum(a.sum() for a in c0) > 0
+
+
+def test_library_streamed_misuse_is_rejected():
+    w = _small_mixed_workload()
+    geoms = W.geometries(w)
+    nt, npts, chunks = W.library_chunks(w, 4, max_tracks=2)
+    chunks = list(chunks)
+    first = chunks[0]
+    n_first = len(first[0])
+    # more tracks than begin sized
+    with pytest.raises(st.SuperTrackError, match="exceed"):
+        st.TrackLibrary.streamed(1, npts, iter([first]), geoms, st.ST_F32)
+    # more points than begin sized
+    with pytest.raises(st.SuperTrackError, match="exceed"):
+        st.TrackLibrary.streamed(nt, n_first - 1, iter([first]), geoms, st.ST_F32)
+    # finish before every track was appended
+    with pytest.raises(st.SuperTrackError, match="appended"):
+        st.TrackLibrary.streamed(nt, npts, iter(chunks[:1]), geoms, st.ST_F32)
+    # a batch over a library that was not finished
+    import ctypes as C
+    from supertrack_amd import _capi
+    h = C.c_void_p()
+    garr = (_capi.Geometry * len(geoms))(*[g.c for g in geoms])
+    st.call("st_library_begin", C.c_int64(nt), C.c_int64(npts), garr, len(geoms), st.ST_F32, C.byref(h))
+    lib = st.TrackLibrary.__new__(st.TrackLibrary)
+    lib.h, lib.geometries = h, geoms
+    hist = st.Histogram(100, 0.1, 10.0, "lin")
+    with pytest.raises(st.SuperTrackError, match="not finished"):
+        st.BatchRun(lib, [hist], 2, 1, True, True)
+    hist.free()
+    lib.free()
+
+
 def test_batch_empty_library():
     geoms = [st.SphericalGeometry(2.5e3, 10.0)]
     z0 = np.zeros(0)
