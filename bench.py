@@ -53,7 +53,7 @@ def parse():
     p.add_argument("--oversamples", type=int, default=0, help="override the workload's oversamples")
     p.add_argument("--no-compare", action="store_true", help="skip the plain-FP64 / batched-FP64 legs")
     p.add_argument("--plain-stride", type=int, default=16, help="plain FP64 leg: every k-th track of cfg3")
-    p.add_argument("--cpu-stride", type=int, default=20, help="CPU oracle sample: every k-th track")
+    p.add_argument("--cpu-stride", type=int, default=10, help="CPU oracle sample: every k-th track")
     p.add_argument("--cpu-oversamples", type=int, default=0, help="CPU sample oversamples (0: workload's)")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--autotune", action="store_true")
