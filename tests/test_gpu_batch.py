@@ -72,8 +72,10 @@ def compare_records(rec, oracle, rtol):
 
 
 def check_bins(counts, edges, oracle, rtol):
-    """Histogram counts equal the oracle's except for near-edge spheres."""
+    """Histogram counts equal the oracle's except for near-edge spheres.
+    Returns (count differences, near-edge sums) summed over the histograms."""
     sums = np.concatenate([s for _, s in oracle.values()]) if oracle else np.zeros(0)
+    tot_diff, tot_near = 0, 0
     for c, ed in zip(counts, edges):
         want = po.histogram_counts(ed, sums)
         diff = np.abs(c.astype(np.int64) - want.astype(np.int64)).sum()
@@ -86,6 +88,9 @@ def check_bins(counts, edges, oracle, rtol):
                     near += 1
                     break
         assert diff <= 2 * near, (diff, near)
+        tot_diff += int(diff)
+        tot_near += near
+    return tot_diff, tot_near
 
 
 @pytest.mark.parametrize("precision,rtol", [(st.ST_F64, 1e-12), (st.ST_F32, 1e-6)])
@@ -103,6 +108,37 @@ def test_batch_cfg3_like_subset(precision, rtol):
     n, worst = compare_records(rec, oracle, rtol)
     assert n > 10000
     check_bins(counts, edges, oracle, rtol)
+
+
+@pytest.mark.parametrize("name,precision,rtol,stride,n_os", [
+    ("cfg3", st.ST_F64, 1e-12, 256, 20),
+    ("cfg3", st.ST_F32, 1e-6, 256, 20),
+    ("cfg4", st.ST_F32, 1e-6, 256, 4),
+    ("cfg5", st.ST_F32, 1e-6, 1024, 10),
+])
+def test_batch_parity_report_per_config(name, precision, rtol, stride, n_os):
+    # every stride-th track of the workload's library (full sizes, up to 1e6 or
+    # 2e6 points), all its geometries and histograms: sphere keys exact, the
+    # worst per-sphere energy error, and how many sums lie within the tolerance
+    # of a bin edge (the only counts allowed to differ)
+    w = W.workload(name)
+    n_sample = w["n_tracks"] // stride
+    x, y, z, e, off, gt = W.make_library(w, n_tracks=n_sample, stride=stride)
+    geo = w["geometries"]
+    mixed = len(geo) > 1
+    geoms = [po.geometry(*g) for g in geo]
+    oracle = oracle_segments(x, y, z, e, off, geoms, gt if mixed else None, n_os, W.SEED, True, True)
+    n_rec = sum(len(k) for k, _ in oracle.values())
+    counts, rec, info, edges = gpu_run(x, y, z, e, off, geo, gt if mixed else None, n_os, W.SEED, True, True,
+                                       precision, w["hists"], record=n_rec + 1024)
+    assert info["exact_tracks"] == 0
+    n, worst = compare_records(rec, oracle, rtol)
+    diff, near = check_bins(counts, edges, oracle, rtol)
+    print(f"\nPARITY {name} {'F64' if precision == st.ST_F64 else 'F32'}: {n_sample} tracks "
+          f"({int(off[-1])} points) x {n_os} oversamples, {n} spheres, max relative energy error "
+          f"{worst:.3e} (tolerance {rtol:g}), {near} sums within tolerance of a bin edge, "
+          f"{diff} bin counts differ", flush=True)
+    assert n > 100000
 
 
 def test_batch_f32_small_spheres_64bit_keys():
