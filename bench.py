@@ -57,6 +57,9 @@ def parse():
     p.add_argument("--cpu-oversamples", type=int, default=0, help="CPU sample oversamples (0: workload's)")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--autotune", action="store_true")
+    p.add_argument("--per-geometry", action="store_true",
+                   help="mixed-geometry workloads: also run each geometry's tracks as its own library, "
+                        "autotuned separately, and report the kernel times beside the one mixed launch")
     p.add_argument("--workers", type=int, default=4,
                    help="plain path: concurrent workers, one HIP stream each (the reference's [Run] Threads)")
     p.add_argument("--via-cli", action="store_true",
@@ -418,6 +421,23 @@ def main():
     log(f"{args.workload} batched: {secs * 1e3:.1f} ms/step, kernel {kms:.1f} ms, {value:.3e} deposits/s, "
         f"params {params}")
     lib.free()
+    per_geometry = None
+    if args.per_geometry and len(w["geometries"]) > 1:
+        per_geometry = {"mixed_kernel_ms": kms, "geometries": []}
+        total_ms = 0.0
+        for g, (hl, dia) in enumerate(w["geometries"]):
+            ng, npg, chunks = W.library_chunks(w, n_tracks, only_geometry=g)
+            lg = st.TrackLibrary.streamed(ng, npg, chunks, [W.geometries(w)[g]], w["precision"])
+            _, kg, pg, _ = time_batch(lg, w, n_os, 1, 1, True, ev)
+            lg.free()
+            total_ms += kg
+            per_geometry["geometries"].append({"half_length": hl, "diameter": dia, "tracks": ng, "points": npg,
+                                               "kernel_ms": kg, "launch": {k: pg[k] for k in (
+                                                   "table_slots", "blocks_per_cu", "points_per_thread")}})
+            log(f"{args.workload} geometry {g} ({hl}, {dia}) alone, autotuned: {ng} tracks, kernel {kg:.1f} ms, "
+                f"{pg['table_slots']} slots x {pg['blocks_per_cu']} wg/CU x {pg['points_per_thread']} pt/thread")
+        per_geometry["sum_kernel_ms"] = total_ms
+        log(f"{args.workload}: one mixed launch {kms:.1f} ms, three autotuned launches {total_ms:.1f} ms")
     key = (args.workload, n_tracks, n_os, params.get("table_slots"), params.get("points_per_thread"))
     traffic = MEASURED_TRAFFIC.get(key)
 
@@ -479,7 +499,8 @@ def main():
                    "fast_tracks": info["fast_tracks"], "exact_tracks": info["exact_tracks"],
                    "scored_fraction": params["points_scored"] / deposits,
                    "fp64_recheck_fraction_of_scored": params["fp64_rechecks"] / max(params["points_scored"], 1),
-                   "in_sphere_fraction": params["points_in_sphere"] / deposits},
+                   "in_sphere_fraction": params["points_in_sphere"] / deposits,
+                   **({"per_geometry": per_geometry} if per_geometry else {})},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "basis": f"{bytes_per_dep:.6g} B per deposit x {deposits} deposits / batch kernel time "

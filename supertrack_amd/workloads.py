@@ -102,22 +102,25 @@ def _spec_for(w: dict, g: SphericalGeometry) -> dict:
     return spec
 
 
-def library_chunks(w: dict, n_tracks: int = None, max_tracks: int = 1024):
+def library_chunks(w: dict, n_tracks: int = None, max_tracks: int = 1024, only_geometry: int = None):
     """The library of make_library(w, n_tracks) as (n_tracks, n_points, chunks): chunks
     yields (x, y, z, edep, offsets, geom_of_track) per run of at most max_tracks
     consecutive same-geometry tracks, generated when it is consumed, so only one
-    run is held on the host at a time."""
+    run is held on the host at a time.  With only_geometry = g, only the tracks
+    of geometry g, as a one-geometry library (geometry index 0)."""
     n_tracks = w["n_tracks"] if n_tracks is None else n_tracks
     geoms = geometries(w)
     runs = _geometry_runs(w, n_tracks, max_tracks)
+    if only_geometry is not None:
+        runs = [r for r in runs if r[2] == only_geometry]
     n_points = int(sum(int(synth_counts(_spec_for(w, geoms[g]), c, t0).sum()) for t0, c, g in runs))
 
     def gen():
         for t0, c, g in runs:
             x, y, z, e, off = synth_library(_spec_for(w, geoms[g]), c, t0)
-            yield x, y, z, e, off, np.full(c, g, np.int32)
+            yield x, y, z, e, off, np.full(c, 0 if only_geometry is not None else g, np.int32)
 
-    return n_tracks, n_points, gen()
+    return sum(c for _, c, _ in runs), n_points, gen()
 
 
 def make_library(w: dict, n_tracks: int = None, t0: int = 0, stride: int = 1):
