@@ -85,24 +85,25 @@ struct GeomDev {
     double r2q;         // r^2 / D^2
     // ST_F64 thresholds in lattice units (as BlockF32's, with the FP64
     // rounding bound dl of the geometry: the data carry no FP32 rounding)
-    double hq_in, hq_out, fm, mrgq;
+    double fm, mrgq;
     unsigned long long N, N2;
 };
 
-// ST_F32 block record (64 B, read through the scalar cache).  The fast path
+// ST_F32 block record (48 B, read through the scalar cache).  The fast path
 // works in lattice units: q = (coordinate - gso) / D, so rint(q) is the
-// sphere index and q - rint(q) the offset from the sphere centre.
+// sphere index and q - rint(q) the offset from the sphere centre; the voxel
+// faces are the half-way planes q = -1/2 and q = N - 1/2, so a point that is
+// not within dl of a half-way plane is inside the voxel iff every rint(q)
+// lies in [0, N - 1].
 //   oq*     (origin - gso) / D of the block's FP32 offsets (z with H' added);
 //   dl      bound on |approximate - reference| of any shifted, offset
 //           coordinate of the block in nm (FP32 rounding of the offsets plus,
 //           with a wide margin, FP64 rounding of every step, any |shift| <= H');
-//   hq_in   |q + gso / D| < hq_in on every axis: inside the voxel;
-//   hq_out  |q + gso / D| > hq_out on any axis: outside;
 //   fm      |q - rint(q)| >= fm: within dl of a half-way plane;
 //   mrgq    |sum (rint(q) - q)^2 - r^2 / D^2| <= mrgq: within the error of
 //           the squared distance.
 struct BlockF32 {
-    double oqx, oqy, oqz, dl, hq_in, hq_out, fm, mrgq;
+    double oqx, oqy, oqz, dl, fm, mrgq;
 };
 
 struct HistDev {
@@ -166,10 +167,8 @@ __device__ __forceinline__ BlockF32 load_block(const BlockF32 *p, int64_t idx)
     r.oqy = q[1];
     r.oqz = q[2];
     r.dl = q[3];
-    r.hq_in = q[4];
-    r.hq_out = q[5];
-    r.fm = q[6];
-    r.mrgq = q[7];
+    r.fm = q[4];
+    r.mrgq = q[5];
     return r;
 }
 
@@ -557,20 +556,23 @@ void batch_kernel(BatchArgs a)
                     const double qx = fma(cur[j].x, g.invD, sqx - g.qc);
                     const double qy = fma(cur[j].y, g.invD, sqy - g.qc);
                     const double qz = fma(cur[j].z, g.invD, (sqz + g.H * g.invD) - g.qc);
-                    const double px = fabs(qx + g.qc), py = fabs(qy + g.qc), pz = fabs(qz + g.qc);
-                    const bool box_in = px < g.hq_in && py < g.hq_in && pz < g.hq_in;
-                    const bool box_out = px > g.hq_out || py > g.hq_out || pz > g.hq_out;
                     const double tx = qx + kRoundMagic, ty = qy + kRoundMagic, tz = qz + kRoundMagic;
                     const double ux = (tx - kRoundMagic) - qx, uy = (ty - kRoundMagic) - qy,
                                  uz = (tz - kRoundMagic) - qz;
                     const double distq = fma(ux, ux, fma(uy, uy, uz * uz));
-                    const bool near = fabs(ux) >= g.fm || fabs(uy) >= g.fm || fabs(uz) >= g.fm ||
-                                      fabs(distq - g.r2q) <= g.mrgq;
-                    const bool amb = valid && !box_out && (!box_in || near);
-                    in = valid && box_in && !near && distq < g.r2q;
-                    key = lattice_key((unsigned)__double2loint(tx), (unsigned)__double2loint(ty),
-                                      (unsigned)__double2loint(tz), (unsigned)g.N);
-                    if (amb) in = exact_member(cur[j].x, cur[j].y, cur[j].z + g.H, g, sh.x, sh.y, sh.z, key);
+                    // low words of t: the indices (|q| < 2^30, checked per track)
+                    const unsigned kx = (unsigned)__double2loint(tx), ky = (unsigned)__double2loint(ty),
+                                   kz = (unsigned)__double2loint(tz);
+                    const bool box = max(max(kx, ky), kz) < (unsigned)g.N;
+                    const bool near = (fmax(fmax(fabs(ux), fabs(uy)), fabs(uz)) >= g.fm) |
+                                      (fabs(distq - g.r2q) <= g.mrgq);
+                    in = valid & box & !near & (distq < g.r2q);
+                    key = lattice_key(kx, ky, kz, (unsigned)g.N);
+                    if (valid & near) {
+                        // redo exactly unless clearly outside (an index beyond [-1, N])
+                        in = max(max(kx + 1u, ky + 1u), kz + 1u) <= (unsigned)g.N + 1u &&
+                             exact_member(cur[j].x, cur[j].y, cur[j].z + g.H, g, sh.x, sh.y, sh.z, key);
+                    }
                     if (in) Acc<ST_F64>::split(cur[j].w / g.denom, v0, v1);
                 } else {
                     // block-uniform terms (scalar loads)
@@ -580,23 +582,25 @@ void batch_kernel(BatchArgs a)
                     const double qx = fma((double)cur[j].x, g.invD, cqx);
                     const double qy = fma((double)cur[j].y, g.invD, cqy);
                     const double qz = fma((double)cur[j].z, g.invD, cqz);
-                    const double px = fabs(qx + g.qc), py = fabs(qy + g.qc), pz = fabs(qz + g.qc);
-                    const bool box_in = px < bk.hq_in && py < bk.hq_in && pz < bk.hq_in;
-                    const bool box_out = px > bk.hq_out || py > bk.hq_out || pz > bk.hq_out;
                     const double tx = qx + kRoundMagic, ty = qy + kRoundMagic, tz = qz + kRoundMagic;
                     const double ux = (tx - kRoundMagic) - qx, uy = (ty - kRoundMagic) - qy,
                                  uz = (tz - kRoundMagic) - qz;
                     const double distq = fma(ux, ux, fma(uy, uy, uz * uz));
-                    const bool near = fabs(ux) >= bk.fm || fabs(uy) >= bk.fm || fabs(uz) >= bk.fm ||
-                                      fabs(distq - g.r2q) <= bk.mrgq;
-                    // any decision within the error bound is redone exactly from FP64
-                    const bool amb = valid && !box_out && (!box_in || near);
-                    in = valid && box_in && !near && distq < g.r2q;
-                    // low words of t hold the (non-negative) indices
-                    key = lattice_key((unsigned)__double2loint(tx), (unsigned)__double2loint(ty),
-                                      (unsigned)__double2loint(tz), (unsigned)g.N);
-                    n_flagged += amb;
-                    if (amb) in = exact_member(a.x64[i], a.y64[i], a.z64[i] + g.H, g, sh.x, sh.y, sh.z, key);
+                    // low words of t: the indices (|q| < 2^30, checked per track)
+                    const unsigned kx = (unsigned)__double2loint(tx), ky = (unsigned)__double2loint(ty),
+                                   kz = (unsigned)__double2loint(tz);
+                    const bool box = max(max(kx, ky), kz) < (unsigned)g.N;
+                    const bool near = (fmax(fmax(fabs(ux), fabs(uy)), fabs(uz)) >= bk.fm) |
+                                      (fabs(distq - g.r2q) <= bk.mrgq);
+                    in = valid & box & !near & (distq < g.r2q);
+                    key = lattice_key(kx, ky, kz, (unsigned)g.N);
+                    if (valid & near) {
+                        // any decision within the error bound is redone exactly
+                        // from FP64, unless clearly outside (an index beyond [-1, N])
+                        const bool cand = max(max(kx + 1u, ky + 1u), kz + 1u) <= (unsigned)g.N + 1u;
+                        n_flagged += cand;
+                        in = cand && exact_member(a.x64[i], a.y64[i], a.z64[i] + g.H, g, sh.x, sh.y, sh.z, key);
+                    }
                     v0 = rint_to_i64((double)cur[j].w * g.yscale);
                 }
                 n_scored += valid;
@@ -1104,6 +1108,14 @@ int st_library_append(st_library *L, const double *x, const double *y, const dou
         const double ylim = (precision == ST_F64) ? 2.7e11 : 1.0e9;
         const double ypoint = (precision == ST_F64) ? kMaxPointY64 : kMaxPointY32;
         bool fast = finite && n <= kMaxFastPoints && ysum < ylim && emax / denom < ypoint;
+        if (fast && n > 0) {
+            // the fast path reads sphere indices from the low words of the
+            // rounded lattice coordinates: |q| < 2^30 for every shift |s| <= H'
+            const double cmax = std::max({std::fabs(xmin), std::fabs(xmax), std::fabs(ymin), std::fabs(ymax),
+                                          std::fabs(sz[o] + g.half_length),
+                                          std::fabs(sz[o + n - 1] + g.half_length)});
+            fast = (cmax + g.half_length + std::fabs(g.gso)) / g.diameter < 1073741824.0;
+        }
         if (fast) {
             // entries kept by a flush come from the next point's layer (or, for
             // keys with an index equal to N, the layer below): a z-window two
@@ -1153,8 +1165,6 @@ int st_library_append(st_library *L, const double *x, const double *y, const dou
                 const double dlq = dl * invD;
                 const double r2q = (g.radius * g.radius) / (g.diameter * g.diameter);
                 bk.dl = dl;
-                bk.hq_in = (g.half_length - dl) * invD;
-                bk.hq_out = (g.half_length + dl) * invD;
                 bk.fm = 0.5 - (dlq + 1e-12);
                 bk.mrgq = 3.0 * dlq * (1.0 + 2.0 * dlq) + 1e-14 * r2q;
             }
@@ -1213,8 +1223,6 @@ int st_library_finish(st_library *L)
         {
             const double dl = kTwoM44 * (7.0 * g.half_length + std::fabs(g.gso) + g.diameter + 16.0);
             const double dlq = dl / g.diameter;
-            gd[i].hq_in = (g.half_length - dl) / g.diameter;
-            gd[i].hq_out = (g.half_length + dl) / g.diameter;
             gd[i].fm = 0.5 - (dlq + 1e-12);
             gd[i].mrgq = 3.0 * dlq * (1.0 + 2.0 * dlq) + 1e-14 * gd[i].r2q;
         }
