@@ -181,34 +181,50 @@ __global__ __launch_bounds__(kScoreBlock) void score_kernel(ScoreArgs a)
         }
     }
 
-    if (tid == 0) {
+    // decoupled look-back by wave 0: each step reads the status words of up to
+    // 64 predecessors at once (lane l: tile - 1 - l - done), sums the counts up
+    // to the nearest inclusive prefix, and waits only if one of those words is
+    // not yet published
+    if (wave == 0) {
         uint32_t excl = 0;
+        const uint32_t ep = a.epoch & 0x3FFFFFFFu;
         if (tile == 0) {
-            __hip_atomic_store(&a.status[0], pack_status(2, a.epoch, tile_total), __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
+            if (lane == 0)
+                __hip_atomic_store(&a.status[0], pack_status(2, a.epoch, tile_total), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
         } else {
-            __hip_atomic_store(&a.status[tile], pack_status(1, a.epoch, tile_total),
-                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            int j = tile - 1;
-            const uint32_t ep = a.epoch & 0x3FFFFFFFu;
+            if (lane == 0)
+                __hip_atomic_store(&a.status[tile], pack_status(1, a.epoch, tile_total), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            int hi = tile - 1;   // nearest predecessor not yet summed
             while (true) {
-                const unsigned long long s =
-                    __hip_atomic_load(&a.status[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                const uint32_t sep = (uint32_t)(s >> 32) & 0x3FFFFFFFu;
-                const unsigned long long fl = s >> 62;
-                if (sep != ep || fl == 0) {
+                const int j = hi - lane;
+                unsigned long long st = 0;
+                if (j >= 0) st = __hip_atomic_load(&a.status[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const bool ready = j >= 0 && ((uint32_t)(st >> 32) & 0x3FFFFFFFu) == ep && (st >> 62) != 0;
+                const uint64_t incl = __ballot(ready && (st >> 62) == 2);
+                const uint64_t waiting = __ballot(j >= 0 && !ready);
+                // lanes summed this step: up to the first inclusive prefix, or all
+                const uint64_t span = incl ? ((2ull << (__ffsll((long long)incl) - 1)) - 1ull) : ~0ull;
+                if (waiting & span) {
                     __builtin_amdgcn_s_sleep(1);
                     continue;
                 }
-                excl += (uint32_t)s;
-                if (fl == 2) break;
-                --j;
+                uint32_t v = (((span >> lane) & 1ull) && j >= 0) ? (uint32_t)st : 0u;
+#pragma unroll
+                for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+                excl += v;
+                if (incl) break;
+                hi -= 64;
             }
-            __hip_atomic_store(&a.status[tile], pack_status(2, a.epoch, excl + tile_total),
-                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (lane == 0)
+                __hip_atomic_store(&a.status[tile], pack_status(2, a.epoch, excl + tile_total), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
         }
-        s_excl = excl;
-        if (tile == a.num_tiles - 1) *a.out_count = (int32_t)(excl + tile_total);
+        if (lane == 0) {
+            s_excl = excl;
+            if (tile == a.num_tiles - 1) *a.out_count = (int32_t)(excl + tile_total);
+        }
     }
     __syncthreads();
     const uint32_t excl = s_excl;

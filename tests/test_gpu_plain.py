@@ -100,6 +100,19 @@ def test_pairs_random_near_boundaries():
     assert_pairs_equal(xs, y, z, e, 2.5e3, 10.0, 5, 0, 1, False, False)
 
 
+def test_pairs_track_of_more_than_64_tiles():
+    # 150,000 points = 147 compaction tiles of 1024: the look-back walks more
+    # than one 64-tile window
+    rng = np.random.default_rng(150)
+    n = 150000
+    H = po.geometry(2.5e3, 10.0).half_length
+    x = rng.uniform(-H, H, n)
+    y = rng.uniform(-H, H, n)
+    z = rng.uniform(-2 * H, 0.0, n)
+    e = rng.uniform(1.0, 100.0, n)
+    assert assert_pairs_equal(x, y, z, e, 2.5e3, 10.0, 21, 3, 3, True, True) > 100000
+
+
 def test_pairs_large_track():
     w = W.workload("cfg3")
     spec = dict(w["synth"], entry_half_length=2500.0, voxel_half_length=2500.0)
