@@ -369,14 +369,14 @@ __device__ void lds_sort_reduce_bin(const HistArgs &a, int m, unsigned char *sme
     const uint64_t lt_mask = (1ull << lane) - 1ull;
     for (int d = lane; d < kBuckets; d += 64) cnt[d * kSortWaves + wave] = 0;
     __syncthreads();
-    // count: per 64-pair step, the lowest lane of each hash adds its peers
-    for (int base = w_lo; base < w_hi; base += 64) {
-        const int i = base + lane;
-        const bool valid = i < w_hi;
-        const uint32_t h = valid ? ph[i] : 0u;
-        const uint64_t peers = hash_peers(h, valid);
-        if (valid && (peers & lt_mask) == 0)
-            cnt[h * kSortWaves + wave] = (uint16_t)(cnt[h * kSortWaves + wave] + __popcll(peers));
+    // count (order does not matter here): one LDS atomic per pair on the
+    // 32-bit word holding its 16-bit (bucket, wave) counter; a counter stays
+    // below 2^16 (at most per_wave pairs), so no carry reaches the other half
+    {
+        unsigned int *cnt32 = reinterpret_cast<unsigned int *>(cnt);
+        const unsigned int inc = 1u << (16 * (wave & 1));
+        for (int i = w_lo + lane; i < w_hi; i += 64)
+            atomicAdd(&cnt32[((unsigned)ph[i] * kSortWaves + wave) >> 1], inc);
     }
     __syncthreads();
     // exclusive scan, bucket-major: all of bucket b's pairs (waves in order)
