@@ -464,6 +464,7 @@ void batch_kernel(BatchArgs a)
             placed[r] = -1;
             if (kk[r] != kEmpty) {
                 unsigned int h = slot_hash<SLOTS>(kk[r]);
+#pragma unroll 1
                 for (int probe = 0; probe < SLOTS; ++probe) {
                     const unsigned long long prev = atomicCAS(&tkey[h], kEmpty, kk[r]);
                     if (prev == kEmpty) break;
@@ -606,12 +607,15 @@ void batch_kernel(BatchArgs a)
                 n_scored += valid;
                 n_in += in;
                 if (in) {
+                    // linear probing; the loop is kept rolled: nearly every
+                    // probe ends at its first slot, and an unrolled loop pays
+                    // exec-mask merges for every possible exit
                     unsigned int h = slot_hash<SLOTS>(key);
-                    unsigned long long prev = 0;
-                    for (int probe = 0; probe < SLOTS; ++probe) {
-                        prev = atomicCAS(&tkey[h], kEmpty, key);
-                        if (prev == kEmpty || prev == key) break;
+                    unsigned long long prev = atomicCAS(&tkey[h], kEmpty, key);
+#pragma unroll 1
+                    for (int probe = 1; probe < SLOTS && prev != kEmpty && prev != key; ++probe) {
                         h = next_slot<SLOTS>(h);
+                        prev = atomicCAS(&tkey[h], kEmpty, key);
                     }
                     if (prev == kEmpty || prev == key) {
                         if (prev == kEmpty) fresh[j] = (int)h;

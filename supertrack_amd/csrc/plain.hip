@@ -255,7 +255,7 @@ constexpr int kBuckets = 1 << kHashBits;
 // per (bucket, wave) counts, bucket-major: thread t scans bucket t's row
 constexpr size_t kCntBytes = (size_t)kBuckets * kSortWaves * sizeof(uint16_t);
 // layout [P hash][P index][counts][Q hash][Q index]: the sorted result lands
-// in Q, and P + counts is one contiguous free region
+// in Q (hashes and pair indices), and P + counts is one contiguous free region
 constexpr size_t kSortBytes = (size_t)kLdsCap * 4 * sizeof(uint16_t) + kCntBytes;
 // free region after the sort: keys and energies in sorted order fit up to
 // kStage pairs
@@ -336,8 +336,9 @@ __device__ __forceinline__ uint64_t hash_peers(uint32_t h, bool valid)
 __device__ void lds_sort_reduce_bin(const HistArgs &a, int m, unsigned char *smem)
 {
     uint16_t *ph = reinterpret_cast<uint16_t *>(smem);
-    uint16_t *pv = ph + kLdsCap;
-    uint16_t *cnt = pv + kLdsCap;   // [kBuckets][kSortWaves]
+    // the P index slot (ph + kLdsCap) is not used by the one-pass sort; with
+    // P hash and the counts it is the staging region after the scatter
+    uint16_t *cnt = ph + 2 * kLdsCap;   // [kBuckets][kSortWaves]
     uint16_t *qh = cnt + kBuckets * kSortWaves;
     uint16_t *qv = qh + kLdsCap;
     uint32_t *sbins = reinterpret_cast<uint32_t *>(smem + kSortBytes);
