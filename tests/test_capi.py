@@ -86,3 +86,40 @@ def test_synth_count_mode_exact_sizes():
     # the counts call and the fill agree track by track
     x1, _, _, _, off1 = st.synth_library(spec, 1, 102)
     assert len(x1) == counts[2]
+
+
+def test_header_is_plain_c99(tmp_path):
+    # the boundary is a C ABI: the header must compile as strict C
+    import shutil
+    import subprocess
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("no gcc")
+    src = tmp_path / "h.c"
+    src.write_text('#include "supertrack_amd.h"\nint main(void) { return 0; }\n')
+    r = subprocess.run([gcc, "-std=c99", "-Wall", "-Wextra", "-Werror", "-pedantic", "-I",
+                        os.path.join(ROOT, "include"), "-c", str(src), "-o", str(tmp_path / "h.o")],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
+def test_library_chunks_reproduce_make_library():
+    # the streamed library's runs, concatenated, are make_library's arrays
+    w = W.workload("cfg5")
+    w["chunk"] = 3
+    w["synth"]["n_max"] = 20000
+    nt, npts, chunks = W.library_chunks(w, 10, max_tracks=2)
+    parts = list(chunks)
+    x, y, z, e, off, gt = W.make_library(w, 10)
+    assert nt == 10 and npts == int(off[-1])
+    assert [len(p[5]) for p in parts] == [2, 1, 2, 1, 2, 1, 1]
+    for k, ref in enumerate((x, y, z, e)):
+        np.testing.assert_array_equal(np.concatenate([p[k] for p in parts]), ref)
+    np.testing.assert_array_equal(np.concatenate([p[5] for p in parts]), gt)
+    counts = np.concatenate([np.diff(p[4]) for p in parts])
+    np.testing.assert_array_equal(counts, np.diff(off))
+    # one geometry alone: its tracks, as a one-geometry library
+    ng, npg, c1 = W.library_chunks(w, 10, only_geometry=1)
+    p1 = list(c1)
+    assert ng == int((gt == 1).sum()) and all((p[5] == 0).all() for p in p1)
+    assert npg == int(sum(off[t + 1] - off[t] for t in range(10) if gt[t] == 1))
