@@ -457,3 +457,36 @@ def test_batch_degenerate_and_fine_lattices(precision, H, D):
         n, _ = compare_records(rec, oracle, rtol)
         assert n > 0
     check_bins(counts, edges, oracle, rtol)
+
+
+def test_batch_full_size_library_invariants():
+    """The full 4096-track, 2 nm FP32 library (5.98e8 points, 1000 oversamples):
+    size-independent properties at the real size -- a pass is deterministic,
+    two launch shapes give identical histograms and statistics, and the
+    statistics are consistent (in-sphere <= scored <= deposits)."""
+    w = W.workload("cfg4")
+    x, y, z, e, off, _ = W.make_library(w)
+    lib = st.TrackLibrary(x, y, z, e, off, W.geometries(w), None, st.ST_F32)
+    del x, y, z, e
+    deposits = int(off[-1]) * w["n_oversamples"]
+    results = []
+    for slots, bpc in [(1536, 5), (1536, 5), (2048, 4)]:
+        hs = [st.Histogram(nb, lo, hi, kind) for kind, nb, lo, hi in w["hists"]]
+        b = st.BatchRun(lib, hs, w["n_oversamples"], W.SEED, True, True, use_graph=True, table_slots=slots,
+                        blocks_per_cu=bpc, points_per_thread=1)
+        b.run()
+        b.sync()
+        results.append(([h.fetch() for h in hs], b.stats()))
+        b.free()
+        for h in hs:
+            h.free()
+    lib.free()
+    (c0, s0), (c1, s1), (c2, s2) = results
+    for a, b2, c in zip(c0, c1, c2):
+        np.testing.assert_array_equal(a, b2)
+        np.testing.assert_array_equal(a, c)
+    assert s0 == s1 == s2
+    assert 0 < s0["points_in_sphere"] <= s0["points_scored"] <= deposits
+    assert s0["fp64_rechecks"] < s0["points_scored"] // 1000
+    total = sum(int(a.sum()) for a in c0)
+    assert 0 < total <= s0["points_in_sphere"]
