@@ -200,6 +200,21 @@ def test_plain_loop_bins_match_oracle_cfg2_subset():
     assert bins[0].sum() > 1000
 
 
+def test_plain_loop_bins_match_oracle_cfg2_full():
+    # the whole 256-track, 100-oversample workload, bin for bin
+    w = W.workload("cfg2")
+    x, y, z, e, off, _ = W.make_library(w)
+    kind, nb, lo, hi = w["hists"][0]
+    h = st.Histogram(nb, lo, hi, kind)
+    g = st.SphericalGeometry(2.5e3, 10.0)
+    n_os = w["n_oversamples"]
+    st.run_plain(x, y, z, e, off, g, h, n_os, W.SEED, True, True)
+    bins, _, _ = po.run(x, y, z, e, off, [po.geometry(2.5e3, 10.0)], None, n_os, W.SEED, True, True,
+                        [po.edges(kind, lo, hi, nb)])
+    np.testing.assert_array_equal(h.counts(), bins[0].astype(np.float64))
+    assert len(off) - 1 == 256 and bins[0].sum() > 1e6
+
+
 def test_plain_loop_is_deterministic():
     w = W.workload("cfg1")
     x, y, z, e, off, _ = W.make_library(w)
