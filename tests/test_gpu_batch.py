@@ -490,3 +490,31 @@ def test_batch_full_size_library_invariants():
     assert s0["fp64_rechecks"] < s0["points_scored"] // 1000
     total = sum(int(a.sum()) for a in c0)
     assert 0 < total <= s0["points_in_sphere"]
+
+
+def test_batch_fp64_whole_cfg3_against_oracle():
+    """The whole 4096-track, 100-oversample FP64 workload (5.98e10 deposits)
+    against the oracle's run driver, bin for bin.  A count may differ only for
+    a sum within 1e-12 of an edge (about 0.02 such sums are expected in the
+    whole run), so at most one sphere may move to a neighbouring bin."""
+    w = W.workload("cfg3")
+    x, y, z, e, off, _ = W.make_library(w)
+    lib = st.TrackLibrary(x, y, z, e, off, W.geometries(w), None, st.ST_F64)
+    hs = [st.Histogram(nb, lo, hi, kind) for kind, nb, lo, hi in w["hists"]]
+    b = st.BatchRun(lib, hs, w["n_oversamples"], W.SEED, True, True, use_graph=True)
+    b.run()
+    b.sync()
+    got = [h.fetch() for h in hs]
+    edges = [h.edges.copy() for h in hs]
+    b.free()
+    for h in hs:
+        h.free()
+    lib.free()
+    want, secs, threads = po.run(x, y, z, e, off, [po.geometry(*w["geometries"][0])], None, w["n_oversamples"],
+                                 W.SEED, True, True, edges)
+    diff = sum(int(np.abs(g.astype(np.int64) - v.astype(np.int64)).sum()) for g, v in zip(got, want))
+    total = sum(int(v.sum()) for v in want)
+    print(f"\nWHOLE cfg3 F64: {total} counts, {diff} differ (oracle {secs:.1f} s on {threads} threads)",
+          flush=True)
+    assert total > 10**8
+    assert diff <= 2
