@@ -368,7 +368,9 @@ __device__ void lds_sort_reduce_bin(const HistArgs &a, int m, unsigned char *sme
     const int w_lo = min(wave * per_wave, m);
     const int w_hi = min(w_lo + per_wave, m);
     const uint64_t lt_mask = (1ull << lane) - 1ull;
-    for (int d = lane; d < kBuckets; d += 64) cnt[d * kSortWaves + wave] = 0;
+    // zero the counts with contiguous 16-byte stores (conflict-free)
+    for (int k = tid; k < (int)(kCntBytes / sizeof(uint4)); k += kSortThreads)
+        reinterpret_cast<uint4 *>(cnt)[k] = make_uint4(0u, 0u, 0u, 0u);
     __syncthreads();
     // count (order does not matter here): one LDS atomic per pair on the
     // 32-bit word holding its 16-bit (bucket, wave) counter; a counter stays
@@ -383,16 +385,24 @@ __device__ void lds_sort_reduce_bin(const HistArgs &a, int m, unsigned char *sme
     // exclusive scan, bucket-major: all of bucket b's pairs (waves in order)
     // precede bucket b + 1's
     {
+        // thread t's row (bucket t, 16 counts of 16 bits) as two 16-byte words
+        static_assert(kSortWaves == 16, "a row is two 16-byte words");
+        uint4 *row = reinterpret_cast<uint4 *>(cnt + tid * kSortWaves);
+        const uint4 r0 = row[0], r1 = row[1];
+        const uint32_t packed[8] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w};
         uint32_t run = 0;
         uint32_t local[kSortWaves];
 #pragma unroll
         for (int w = 0; w < kSortWaves; ++w) {
             local[w] = run;
-            run += cnt[tid * kSortWaves + w];
+            run += (packed[w >> 1] >> (16 * (w & 1))) & 0xFFFFu;
         }
         const uint32_t base = block_exclusive_scan_1024(run, s_wave);
+        uint32_t out[8];
 #pragma unroll
-        for (int w = 0; w < kSortWaves; ++w) cnt[tid * kSortWaves + w] = (uint16_t)(base + local[w]);
+        for (int k = 0; k < 8; ++k) out[k] = ((base + local[2 * k]) & 0xFFFFu) | ((base + local[2 * k + 1]) << 16);
+        row[0] = make_uint4(out[0], out[1], out[2], out[3]);
+        row[1] = make_uint4(out[4], out[5], out[6], out[7]);
     }
     __syncthreads();
     // scatter in pair order: rank among lower lanes of the same hash, then
