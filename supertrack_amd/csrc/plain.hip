@@ -252,7 +252,8 @@ constexpr int kSortWaves = kSortThreads / 64;
 constexpr int kLdsCap = 12288;
 constexpr int kHashBits = 10;
 constexpr int kBuckets = 1 << kHashBits;
-// per (bucket, wave) counts, bucket-major: thread t scans bucket t's row
+// per (bucket, wave) counts, stored wave-major ([wave][bucket]: random
+// buckets of one wave fall in different LDS banks); thread t scans bucket t
 constexpr size_t kCntBytes = (size_t)kBuckets * kSortWaves * sizeof(uint16_t);
 // layout [P hash][P index][counts][Q hash][Q index]: the sorted result lands
 // in Q (hashes and pair indices), and P + counts is one contiguous free region
@@ -338,7 +339,7 @@ __device__ void lds_sort_reduce_bin(const HistArgs &a, int m, unsigned char *sme
     uint16_t *ph = reinterpret_cast<uint16_t *>(smem);
     // the P index slot (ph + kLdsCap) is not used by the one-pass sort; with
     // P hash and the counts it is the staging region after the scatter
-    uint16_t *cnt = ph + 2 * kLdsCap;   // [kBuckets][kSortWaves]
+    uint16_t *cnt = ph + 2 * kLdsCap;   // [kSortWaves][kBuckets]
     uint16_t *qh = cnt + kBuckets * kSortWaves;
     uint16_t *qv = qh + kLdsCap;
     uint32_t *sbins = reinterpret_cast<uint32_t *>(smem + kSortBytes);
@@ -363,7 +364,7 @@ __device__ void lds_sort_reduce_bin(const HistArgs &a, int m, unsigned char *sme
         }
     }
     for (int i = tid; i < m; i += kSortThreads) ph[i] = key_bucket(a.keys[i], kHashBits);
-    // wave w owns pairs [w_lo, w_hi) and row entries cnt[* x kSortWaves + w]
+    // wave w owns pairs [w_lo, w_hi) and the counts cnt[w x kBuckets + *]
     const int per_wave = (m + kSortWaves - 1) / kSortWaves;
     const int w_lo = min(wave * per_wave, m);
     const int w_hi = min(w_lo + per_wave, m);
@@ -373,36 +374,29 @@ __device__ void lds_sort_reduce_bin(const HistArgs &a, int m, unsigned char *sme
         reinterpret_cast<uint4 *>(cnt)[k] = make_uint4(0u, 0u, 0u, 0u);
     __syncthreads();
     // count (order does not matter here): one LDS atomic per pair on the
-    // 32-bit word holding its 16-bit (bucket, wave) counter; a counter stays
+    // 32-bit word holding its 16-bit (wave, bucket) counter; a counter stays
     // below 2^16 (at most per_wave pairs), so no carry reaches the other half
     {
-        unsigned int *cnt32 = reinterpret_cast<unsigned int *>(cnt);
-        const unsigned int inc = 1u << (16 * (wave & 1));
-        for (int i = w_lo + lane; i < w_hi; i += 64)
-            atomicAdd(&cnt32[((unsigned)ph[i] * kSortWaves + wave) >> 1], inc);
+        unsigned int *cnt32 = reinterpret_cast<unsigned int *>(cnt + wave * kBuckets);
+        for (int i = w_lo + lane; i < w_hi; i += 64) {
+            const unsigned h = ph[i];
+            atomicAdd(&cnt32[h >> 1], 1u << (16 * (h & 1u)));
+        }
     }
     __syncthreads();
     // exclusive scan, bucket-major: all of bucket b's pairs (waves in order)
     // precede bucket b + 1's
     {
-        // thread t's row (bucket t, 16 counts of 16 bits) as two 16-byte words
-        static_assert(kSortWaves == 16, "a row is two 16-byte words");
-        uint4 *row = reinterpret_cast<uint4 *>(cnt + tid * kSortWaves);
-        const uint4 r0 = row[0], r1 = row[1];
-        const uint32_t packed[8] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w};
         uint32_t run = 0;
         uint32_t local[kSortWaves];
 #pragma unroll
         for (int w = 0; w < kSortWaves; ++w) {
             local[w] = run;
-            run += (packed[w >> 1] >> (16 * (w & 1))) & 0xFFFFu;
+            run += cnt[w * kBuckets + tid];
         }
         const uint32_t base = block_exclusive_scan_1024(run, s_wave);
-        uint32_t out[8];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) out[k] = ((base + local[2 * k]) & 0xFFFFu) | ((base + local[2 * k + 1]) << 16);
-        row[0] = make_uint4(out[0], out[1], out[2], out[3]);
-        row[1] = make_uint4(out[4], out[5], out[6], out[7]);
+        for (int w = 0; w < kSortWaves; ++w) cnt[w * kBuckets + tid] = (uint16_t)(base + local[w]);
     }
     __syncthreads();
     // scatter in pair order: rank among lower lanes of the same hash, then
@@ -413,12 +407,12 @@ __device__ void lds_sort_reduce_bin(const HistArgs &a, int m, unsigned char *sme
         const uint32_t h = valid ? ph[i] : 0u;
         const uint64_t peers = hash_peers(h, valid);
         uint32_t cur = 0;
-        if (valid) cur = cnt[h * kSortWaves + wave];
+        if (valid) cur = cnt[wave * kBuckets + h];
         if (valid) {
             const uint32_t pos = cur + (uint32_t)__popcll(peers & lt_mask);
             qh[pos] = (uint16_t)h;
             qv[pos] = (uint16_t)i;
-            if ((peers & lt_mask) == 0) cnt[h * kSortWaves + wave] = (uint16_t)(cur + __popcll(peers));
+            if ((peers & lt_mask) == 0) cnt[wave * kBuckets + h] = (uint16_t)(cur + __popcll(peers));
         }
     }
     __syncthreads();
