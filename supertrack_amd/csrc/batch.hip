@@ -638,8 +638,13 @@ void batch_kernel(BatchArgs a)
             const int64_t next = c + (int64_t)kPPT * kBlk;
             if (++since < kCheck || next >= p_end) continue;
             since = 0;
+            // every wave reads the occupancy before any wave appends again:
+            // a wave that went on past the check could otherwise raise s_occ
+            // under a slower one, which would then flush alone
             __syncthreads();
-            if (s_occ >= kFlushAt) {
+            const bool full = s_occ >= kFlushAt;
+            __syncthreads();
+            if (full) {
                 // every later point lies in a layer >= the next point's exact
                 // layer index (z order), and a key's layer (key / N^2) is never
                 // below its point's layer: entries with smaller keys are complete
