@@ -35,7 +35,7 @@ HBM_PEAK_GBS = 8000.0
 # KiB that a plain 1 GiB copy reads (scale 2.0); WRITE_SIZE reads 1 KiB per KiB
 # written (scale 1.0).  Other workloads or launch shapes report null.
 MEASURED_TRAFFIC = {
-    ("cfg4", 4096, 1000, 1536, 1): 2.0 * 28290580.0 * 1024 + 1.0 * 147332.8 * 1024,
+    ("cfg4", 4096, 1000, 1536, 1): 2.0 * 23262561.4 * 1024 + 1.0 * 147335.6 * 1024,
 }
 
 
