@@ -156,7 +156,8 @@ def test_histogram_tests_known_answers(capacity):
                                                  (20000, 3, None), (12288, 1 << 20, None), (12289, 1 << 40, None),
                                                  (12000, 1, None), (9000, 1 << 62, 12288),
                                                  (3000, 2, None), (4096, 1 << 50, None), (4097, 4, None),
-                                                 (5120, 5, None), (5121, 3, None)])
+                                                 (5120, 5, None), (5121, 3, None), (1536, 1 << 30, None),
+                                                 (1537, 5, None), (1000, 2, None)])
 def test_histogram_random_pairs_match_oracle(m, keyspace, capacity):
     rng = np.random.default_rng(m + 3)
     keys = rng.integers(0, keyspace, m, dtype=np.uint64)
