@@ -28,19 +28,52 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-
-# HBM bytes per launch of the batched kernel, measured with rocprofv3 --pmc
-# FETCH_SIZE and WRITE_SIZE on the workload named by the key (workload, tracks,
-# oversamples, table slots, points per thread).  FETCH_SIZE reads 0.5 KiB per
-# KiB that a plain 1 GiB copy reads (scale 2.0); WRITE_SIZE reads 1 KiB per KiB
-# written (scale 1.0).  Other workloads or launch shapes report null.
-MEASURED_TRAFFIC = {
-    ("cfg4", 4096, 1000, 1536, 1): 2.0 * 23262561.4 * 1024 + 1.0 * 147335.6 * 1024,
-}
+# VALU issue peak of MI355X: 256 CUs x 4 SIMDs, one wave64 VALU instruction
+# per SIMD every 4 cycles, 2.4 GHz peak engine clock.
+VALU_PEAK_GINSTR = 256 * 4 * 2.4e9 / 4 / 1e9
+# Kernel sources whose hash keys the committed counter summaries: counters
+# measured on other sources are not used.
+KERNEL_SOURCES = ["supertrack_amd/csrc/batch.hip", "supertrack_amd/csrc/st_common.hpp",
+                  "supertrack_amd/csrc/st_device.hpp", "supertrack_amd/csrc/st_internal.hpp"]
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def kernel_source_hash():
+    import hashlib
+    h = hashlib.sha256()
+    for rel in KERNEL_SOURCES:
+        with open(os.path.join(ROOT, rel), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
+def load_counters(workload, n_tracks, n_os, params):
+    """Per-launch counters of the batched kernel from the committed rocprofv3
+    summary profiles/batch_kernel_<workload>_counters.txt ("key = value"
+    lines), if it was measured on this workload, launch shape and kernel
+    source.  Returns (dict, path) or (None, reason)."""
+    path = os.path.join("profiles", f"batch_kernel_{workload}_counters.txt")
+    full = os.path.join(ROOT, path)
+    if not os.path.exists(full):
+        return None, f"no {path}"
+    c = {}
+    with open(full) as f:
+        for line in f:
+            line = line.strip()
+            if not line or line.startswith("#") or "=" not in line:
+                continue
+            k, v = (t.strip() for t in line.split("=", 1))
+            c[k] = v
+    want = {"workload": workload, "tracks": str(n_tracks), "oversamples": str(n_os),
+            "table_slots": str(params["table_slots"]), "blocks_per_cu": str(params["blocks_per_cu"]),
+            "points_per_thread": str(params["points_per_thread"]), "kernel_source_sha": kernel_source_hash()}
+    for k, v in want.items():
+        if c.get(k) != v:
+            return None, f"{path} was measured with {k} = {c.get(k)}, this run has {v}"
+    return c, path
 
 
 def parse():
@@ -62,6 +95,8 @@ def parse():
                         "autotuned separately, and report the kernel times beside the one mixed launch")
     p.add_argument("--workers", type=int, default=4,
                    help="plain path: concurrent workers, one HIP stream each (the reference's [Run] Threads)")
+    p.add_argument("--queued", action="store_true",
+                   help="plain workloads: time the method with queued oversamples (one launch per track)")
     p.add_argument("--via-cli", action="store_true",
                    help="plain workloads: also time the C++ driver (supertrack CLI) on the same tracks")
     p.add_argument("--copy-probe", action="store_true",
@@ -159,7 +194,7 @@ def time_batch(lib, w, n_os, steps, warmup, autotune, ev):
     return secs, float(np.mean(kernel_ms)), params, counts
 
 
-def time_plain(w, host, stride, n_os, workers=1):
+def time_plain(w, host, stride, n_os, workers=1, queued=False):
     """Plain FP64 drop-in path: per track begin, n_os x (process, sort-reduce-bin), end.
 
     Tracks are split into contiguous ranges over `workers` workers (the
@@ -195,6 +230,8 @@ def time_plain(w, host, stride, n_os, workers=1):
         methods = [st.VoxelConstrainedSphereMethod(g, True, True) for g in geoms]
         for m in methods:
             m.set_stream(stream)
+            if queued:
+                m.set_queued(True)
         state.append(dict(stream=stream, h=h, methods=methods, trk=st.Track(capacity=cap),
                           pairs=st.VolumeEdepPair(cap), err=None))
     st.synchronize()
@@ -254,7 +291,12 @@ def time_plain(w, host, stride, n_os, workers=1):
     return deposits, secs, len(tracks), counts
 
 
-def cpu_baseline(w, stride, n_os):
+def cpu_baseline(w, stride, n_os, rtol=None):
+    """The FP64 oracle (OpenMP over (track, oversample) segments) on every
+    stride-th track of the workload's library; sample track j uses shift
+    counter j, as the batched run of the same sample does.  Returns (rate,
+    threads, n_sample, deposits, seconds, sample) with sample = (host arrays,
+    oracle bins, near-edge counts at rtol)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
     from supertrack_amd import workloads as W
@@ -262,17 +304,78 @@ def cpu_baseline(w, stride, n_os):
     x, y, z, e, off, gt = W.make_library(w, n_tracks=n_sample, t0=0, stride=stride)
     geoms = [po.geometry(h, d) for h, d in w["geometries"]]
     edges = [po.edges(kind, lo, hi, nb) for kind, nb, lo, hi in w["hists"]]
-    # tracks of the sample keep their library index for the shift stream
     deposits = int(off[-1]) * n_os
-    total = 0.0
-    threads = 0
-    for j in range(n_sample):
-        sub = slice(int(off[j]), int(off[j + 1]))
-        o1 = np.array([0, int(off[j + 1] - off[j])], np.int64)
-        _, secs, threads = po.run(x[sub], y[sub], z[sub], e[sub], o1, geoms, gt[j:j + 1], n_os, W.SEED,
-                                  True, True, edges, track_index_base=j * stride, n_threads=0)
-        total += secs
-    return deposits / total, threads, n_sample, deposits, total
+    mixed = len(geoms) > 1
+    res = po.run(x, y, z, e, off, geoms, gt if mixed else None, n_os, W.SEED, True, True, edges, n_threads=0,
+                 near_rtol=rtol)
+    bins, secs, threads = res[:3]
+    near = res[3] if rtol is not None else None
+    return deposits / secs, threads, n_sample, deposits, secs, ((x, y, z, e, off, gt), bins, near)
+
+
+def sample_parity(w, sample, n_os, rtol, stride, record_tracks=4, record_oversamples=4):
+    """The CPU sample through the batched kernel: every bin against the
+    oracle's (a count may differ only for a sum within rtol of an edge), and
+    the largest per-sphere energy error on the recorded spheres of the first
+    record_tracks sample tracks x record_oversamples oversamples."""
+    import supertrack_amd as st
+    import pyoracle as po
+    from supertrack_amd import workloads as W
+    (x, y, z, e, off, gt), want, near = sample
+    mixed = len(w["geometries"]) > 1
+    lib = st.TrackLibrary(x, y, z, e, off, W.geometries(w), gt if mixed else None, w["precision"])
+    hs = [st.Histogram(nb, lo, hi, kind) for kind, nb, lo, hi in w["hists"]]
+    b = st.BatchRun(lib, hs, n_os, W.SEED, True, True, use_graph=False)
+    b.run()
+    b.sync()
+    bal = b.balance()
+    got = [h.fetch() for h in hs]
+    b.free()
+    lib.free()
+    for h in hs:
+        h.free()
+    diff = [int(np.abs(g.astype(np.int64) - v.astype(np.int64)).sum()) for g, v in zip(got, want)]
+    # energy error on a recorded subset (keys must match exactly)
+    k = min(record_tracks, len(off) - 1)
+    sub = slice(0, int(off[k]))
+    lib = st.TrackLibrary(x[sub], y[sub], z[sub], e[sub], off[:k + 1], W.geometries(w),
+                          gt[:k] if mixed else None, w["precision"])
+    hs = [st.Histogram(nb, lo, hi, kind) for kind, nb, lo, hi in w["hists"]]
+    b = st.BatchRun(lib, hs, record_oversamples, W.SEED, True, True, use_graph=False)
+    cap = int(off[k]) * record_oversamples + 1024
+    b.record(cap)
+    b.run()
+    b.sync()
+    seg, keys, sums = b.record_fetch(cap)
+    b.free()
+    lib.free()
+    for h in hs:
+        h.free()
+    worst, n_sph, keys_ok = 0.0, 0, True
+    geoms = [po.geometry(h, d) for h, d in w["geometries"]]
+    for t in range(k):
+        g = geoms[int(gt[t]) if mixed else 0]
+        segs = po.track_segments(x[off[t]:off[t + 1]], y[off[t]:off[t + 1]], z[off[t]:off[t + 1]],
+                                 e[off[t]:off[t + 1]], g, W.SEED, t, record_oversamples, True, True)
+        for o, (_, _, wk, ws) in enumerate(segs):
+            sel = seg == t * record_oversamples + o
+            order = np.argsort(keys[sel], kind="stable")
+            gk, gs = keys[sel][order], sums[sel][order]
+            if not np.array_equal(gk, wk):
+                keys_ok = False
+                continue
+            if len(ws):
+                worst = max(worst, float((np.abs(gs - ws) / np.abs(ws)).max()))
+            n_sph += len(ws)
+    ok = keys_ok and all(d <= 2 * n for d, n in zip(diff, near)) and worst <= rtol and \
+        bal["energy_in"] == bal["energy_out"] and bal["entries_created"] == bal["entries_retired"]
+    return {"sample": f"every {stride}th track ({len(off) - 1} tracks) x {n_os} oversamples, all histograms",
+            "bins_differ": diff, "near_edge": near,
+            "rule": f"bins_differ <= 2 x near_edge (sums within {rtol:g} of an edge may change bin)",
+            "max_rel_energy_error": worst, "tolerance": rtol,
+            "energy_sample": f"first {k} sample tracks x {record_oversamples} oversamples, {n_sph} spheres, "
+                             f"keys {'identical' if keys_ok else 'DIFFER'}",
+            "conservation": bal, "ok": bool(ok)}
 
 
 def copy_probe():
@@ -298,27 +401,44 @@ def plain_main(args, w, n_tracks, n_os):
     total_points = int(host[4][-1])
     deposits = total_points * n_os
     for _ in range(args.warmup):
-        time_plain(w, host, 1, n_os, args.workers)
+        time_plain(w, host, 1, n_os, args.workers, args.queued)
     times = []
     for _ in range(args.steps):
-        d, secs, nt, counts = time_plain(w, host, 1, n_os, args.workers)
+        d, secs, nt, counts = time_plain(w, host, 1, n_os, args.workers, args.queued)
         times.append(secs)
     secs = float(np.mean(times))
     value = deposits / secs
-    log(f"{args.workload} plain: {secs * 1e3:.1f} ms/step, {value:.3e} deposits/s")
+    mode = "queued oversamples (one launch per track)" if args.queued else "one launch sequence per oversample"
+    log(f"{args.workload} plain, {mode}: {secs * 1e3:.1f} ms/step, {value:.3e} deposits/s")
     extra = {}
+    if not args.queued:
+        # the same tracks with queued oversamples, for comparison
+        time_plain(w, host, 1, n_os, args.workers, True)
+        _, qs, _, qcounts = time_plain(w, host, 1, n_os, args.workers, True)
+        extra["queued_ms_per_step"] = qs * 1e3
+        extra["queued_deposits_per_s"] = deposits / qs
+        extra["queued_counts_equal"] = bool(np.array_equal(qcounts, counts))
+        log(f"{args.workload} plain, queued oversamples: {qs * 1e3:.1f} ms, {deposits / qs:.3e} deposits/s")
     if args.via_cli:
-        ms = time_cli(w, host, n_os, args.workers)
-        extra = {"cli_run_ms": ms, "cli_deposits_per_s": deposits / (ms * 1e-3),
-                 "cli_note": f"supertrack CLI, {args.workers} workers: track file reads (prefetched), "
-                             "per-track loops and the output file, in the program's own clock"}
-        log(f"{args.workload} plain via CLI: {ms:.1f} ms, {deposits / (ms * 1e-3):.3e} deposits/s")
+        for queued in (False, True):
+            ms = time_cli(w, host, n_os, args.workers, queued)
+            tag = "cli_queued" if queued else "cli"
+            extra.update({f"{tag}_run_ms": ms, f"{tag}_deposits_per_s": deposits / (ms * 1e-3)})
+            log(f"{args.workload} plain via CLI{' (QueueOversamples)' if queued else ''}: {ms:.1f} ms, "
+                f"{deposits / (ms * 1e-3):.3e} deposits/s")
+        extra["cli_note"] = (f"supertrack CLI, {args.workers} workers: track file reads (prefetched), per-track "
+                             "loops and the output file, in the program's own clock")
     cpu = None
+    parity = None
     if not args.no_cpu:
-        rate, threads, ns, cdep, csecs = cpu_baseline(w, 1, n_os)
+        rate, threads, ns, cdep, csecs, sample = cpu_baseline(w, 1, n_os)
         cpu = {"value": rate, "unit": "deposits/s", "cores": threads, "kind": "port",
                "sample": f"{args.workload}: all {ns} tracks x {n_os} oversamples, {cdep:.3e} deposits in "
                          f"{csecs:.2f}s, FP64 oracle, OpenMP"}
+        want = sample[1][0]
+        diff = int(np.abs(counts.astype(np.int64) - want.astype(np.int64)).sum())
+        parity = {"sample": f"all {ns} tracks x {n_os} oversamples (the timed workload)", "bins_differ": diff,
+                  "rule": "bit-exact: bins_differ == 0", "ok": diff == 0}
     out = {
         "metric": "deposits scored/s on 1 MI355X, plain FP64 per-track path",
         "value": value, "unit": "deposits/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -326,17 +446,19 @@ def plain_main(args, w, n_tracks, n_os):
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"{args.workload}: {n_tracks} tracks, {total_points} points, {n_os} oversamples, "
                                f"geometries {w['geometries']}, histograms {w['hists'][:1]}",
-                   "deposits_per_step": deposits, "parallelism": f"1 GPU, {args.workers} workers (one stream each)"},
+                   "deposits_per_step": deposits, "parallelism": f"1 GPU, {args.workers} workers (one stream each)",
+                   "mode": mode},
         "roofline": {"bound": "hbm", "achieved": deposits * 32 / secs / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": deposits * 32 / secs / 1e9 / HBM_PEAK_GBS, "traffic": None,
                      "basis": "32 B per deposit / host-clock step time (per-track launch sequences)"},
         "cpu_baseline": cpu,
+        "parity": parity,
     }
     out.update(extra)
     print(json.dumps(out))
 
 
-def time_cli(w, host, n_os, workers):
+def time_cli(w, host, n_os, workers, queued=False):
     """Runs the C++ driver (supertrack CLI) on the workload's tracks written
     as a track file; returns the run time it reports (ms)."""
     import subprocess
@@ -369,6 +491,7 @@ ScoringRegionHalfLength = {H}
 ScoringSphereDiameter = {D}
 ShiftTrack = true
 ShiftTrackZ = true
+QueueOversamples = {"true" if queued else "false"}
 
 [Histogram]
 Type = {kind}
@@ -438,8 +561,35 @@ def main():
                 f"{pg['table_slots']} slots x {pg['blocks_per_cu']} wg/CU x {pg['points_per_thread']} pt/thread")
         per_geometry["sum_kernel_ms"] = total_ms
         log(f"{args.workload}: one mixed launch {kms:.1f} ms, three autotuned launches {total_ms:.1f} ms")
-    key = (args.workload, n_tracks, n_os, params.get("table_slots"), params.get("points_per_thread"))
-    traffic = MEASURED_TRAFFIC.get(key)
+    # roofline: the kernel issues VALU instructions most of its time (it culls
+    # most deposits without reading them and re-reads the rest from L2), so
+    # the bound is VALU issue; HBM traffic and the algorithmic byte rate are
+    # reported beside it.  Counters come from the committed rocprofv3 summary
+    # of this same command, the time from this run's HIP events.
+    counters, cref = load_counters(args.workload, n_tracks, n_os, params)
+    roof = {"bound": "valu", "unit": "G VALU instr/s", "peak": VALU_PEAK_GINSTR, "achieved": None, "frac": None,
+            "traffic": None, "counters": cref,
+            "algorithmic_gbs": achieved,
+            "algorithmic_basis": f"{bytes_per_dep:.6g} B per deposit x {deposits} deposits / kernel time: "
+                                 "informational (most deposits are culled unread, the rest re-read from L2)"}
+    if counters:
+        valu = float(counters["SQ_INSTS_VALU"])
+        traffic = (float(counters["FETCH_SIZE_KiB"]) * float(counters["fetch_scale"]) +
+                   float(counters["WRITE_SIZE_KiB"]) * float(counters["write_scale"])) * 1024.0
+        ach = valu / (kms * 1e-3) / 1e9
+        roof.update({
+            "achieved": ach, "frac": ach / VALU_PEAK_GINSTR, "traffic": traffic,
+            "basis": f"SQ_INSTS_VALU {valu:.6g} per launch ({cref}) / {kms:.2f} ms kernel time (average of "
+                     f"{args.steps} launches, HIP events on the batch stream); peak = 1024 SIMDs x 2.4 GHz / "
+                     "4 cycles per wave64 VALU instruction",
+            "valu_lane_instr_per_scored_point": valu * 64 / max(params["points_scored"], 1),
+            "hbm": {"achieved_gbs": traffic / (kms * 1e-3) / 1e9, "peak_gbs": HBM_PEAK_GBS,
+                    "frac": traffic / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                    "basis": "(FETCH_SIZE x fetch_scale + WRITE_SIZE x write_scale) per launch, scales from a "
+                             "1 GiB copy"},
+        })
+    else:
+        log(f"roofline counters not used: {cref}")
 
     extra = {}
     if not args.no_compare:
@@ -472,13 +622,18 @@ def main():
             f"({pd:.3e} deposits in {ps:.1f}s)")
 
     cpu = None
+    parity = None
     if not args.no_cpu:
         cos = args.cpu_oversamples or n_os
-        rate, threads, ns, cdep, csecs = cpu_baseline(w, args.cpu_stride, cos)
+        rtol = 1e-6 if w["precision"] == st.ST_F32 else 1e-12
+        rate, threads, ns, cdep, csecs, sample = cpu_baseline(w, args.cpu_stride, cos, rtol)
         cpu = {"value": rate, "unit": "deposits/s", "cores": threads, "kind": "port",
                "sample": f"{args.workload}: every {args.cpu_stride}th track ({ns} tracks) x {cos} oversamples, "
                          f"{cdep:.3e} deposits in {csecs:.1f}s, FP64 oracle, OpenMP"}
         log(f"cpu oracle: {rate:.3e} deposits/s on {threads} threads")
+        parity = sample_parity(w, sample, cos, rtol, args.cpu_stride)
+        log(f"parity on the cpu sample: {parity}")
+        del sample
 
     out = {
         "metric": "deposits scored/s on 1 MI355X, batched-FP32 vs plain-FP64; % of HBM peak",
@@ -501,11 +656,9 @@ def main():
                    "fp64_recheck_fraction_of_scored": params["fp64_rechecks"] / max(params["points_scored"], 1),
                    "in_sphere_fraction": params["points_in_sphere"] / deposits,
                    **({"per_geometry": per_geometry} if per_geometry else {})},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "basis": f"{bytes_per_dep:.6g} B per deposit x {deposits} deposits / batch kernel time "
-                              f"({kms:.2f} ms average over {args.steps} launches, HIP events on the batch stream)"},
+        "roofline": roof,
         "cpu_baseline": cpu,
+        "parity": parity,
     }
     out.update(extra)
     print(json.dumps(out))

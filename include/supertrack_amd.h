@@ -70,6 +70,19 @@ int st_device_synchronize(void);
 /* The stream that objects use unless given another (one per device, created
  * by the library on first use, non-blocking). */
 int st_default_stream(void **hip_stream);
+/* Waits for the work on the library's default streams and destroys them, so
+ * that nothing of the library outlives the HIP runtime at process exit.  Call
+ * it after destroying every object, before the process exits (the Python
+ * binding and the supertrack driver do).  A later call that needs a default
+ * stream creates a new one.  No counterpart in the reference (its CUDA
+ * objects are left to the driver at exit). */
+int st_shutdown(void);
+/* Test switch for the allocation-failure paths: with countdown = k > 0 the
+ * k-th device or pinned-host allocation that the library makes from now on
+ * fails as if the device were full (the call reports ST_ENOMEM and releases
+ * what it had allocated); 0 turns the switch off.  No counterpart in the
+ * reference. */
+int st_debug_fail_allocation(int64_t countdown);
 
 /* ---- SphericalGeometry (SphericalGeometry.cuh:8-22) ---- */
 typedef struct {
@@ -100,16 +113,26 @@ void st_track_free(st_track *t);
 typedef struct {
     uint64_t *volume;        /* sphere keys, device                        */
     double *edep;            /* lineal energy per pair (keV/um), device    */
-    int32_t *num_elements;   /* device counter, written by process_oversample */
+    int32_t *num_elements;   /* pair count in pinned, device-mapped host memory
+                                (the reference's managed numElements): the
+                                host reads it after the producing stream has
+                                been synchronised (st_vcs_end_track)      */
     int64_t capacity;
     uint64_t key_bound;      /* largest key the producer can write; 0 = unknown
                                 (set by st_vcs_process_oversample; the radix
                                 sort for more than 12288 pairs runs only the
                                 passes this needs) */
+    int32_t *count_dev;      /* device copy of the count, written next to
+                                num_elements by the producer and read by the
+                                histogram kernels                           */
 } st_pairs;
 
+/* Both counts start at the capacity (VolumeEdepPair.cu:10-13). */
 int st_pairs_alloc(st_pairs *p, int64_t capacity);
 void st_pairs_free(st_pairs *p);
+/* Sets both counts, for pairs that the caller filled itself (waits for the
+ * device first). */
+int st_pairs_set_count(st_pairs *p, int32_t count);
 
 /* ---- VoxelConstrainedSphereMethod (the per-track drop-in) ---- */
 typedef struct st_vcs st_vcs;
@@ -129,6 +152,21 @@ int st_vcs_begin_track(st_vcs *m, st_track *track, uint64_t seed, uint32_t track
 int st_vcs_process_oversample(st_vcs *m, const st_track *track, st_pairs *out);
 int st_vcs_end_track(st_vcs *m);
 void st_vcs_destroy(st_vcs *m);
+/* Queued oversamples (on != 0; off by default).  The reference's per-track
+ * loop reads nothing between its per-oversample calls before the track's
+ * synchronisation (SuperTrackManager.cu:71-76), so a queued method records
+ * st_vcs_process_oversample calls, histograms record their st_hist_add calls
+ * on the same pairs, and st_vcs_end_track runs the whole track in one launch
+ * (one workgroup per oversample: score, compaction, sort, reduce, bin).
+ * Results equal the per-oversample path's bit for bit, and after the track
+ * ends the pairs hold the last oversample's, as they would.  Until then the
+ * pairs' contents and counts are undefined.  Pending work also runs when it
+ * is needed: st_hist_end_track / st_hist_fetch / st_hist_begin_track /
+ * st_hist_set_stream of a consuming histogram, st_pairs_free of the pairs, a
+ * new track, another pairs object, or turning the mode off.  Tracks of more
+ * than 12288 points (or whose queued pair rows would exceed 1 GiB) run
+ * oversample by oversample.  No counterpart in the reference. */
+int st_vcs_set_queued(st_vcs *m, int32_t on);
 
 /* ---- Histogram ---- */
 typedef struct st_hist st_hist;
@@ -224,6 +262,14 @@ int st_batch_params(const st_batch *b, int32_t *params4);
  * and was redone from the FP64 side copy (ST_F32), stats3[2] points inside a
  * sphere.  Waits for the run.  No counterpart in the reference: measurement. */
 int st_batch_stats(st_batch *b, int64_t *stats3);
+/* Conservation totals of the last st_batch_run's fused kernel (waits for the
+ * run), all modulo 2^64: balance4[0] the fixed-point energy words inserted
+ * into the sphere tables, [1] those of the entries retired from them (binned
+ * or outside every histogram), [2] table entries created, [3] entries
+ * retired.  A pass that loses no entry has [0] == [1] and [2] == [3];
+ * st_batch_sync returns ST_ESTATE otherwise.  No counterpart in the
+ * reference: a check on the fused kernel. */
+int st_batch_balance(st_batch *b, uint64_t *balance4);
 /* Test hook: record (segment, key, sum) of every scored sphere of the next
  * st_batch_run into host-visible arrays of the given capacity (segment =
  * track * n_oversamples + oversample).  capacity 0 disables recording. */
@@ -231,6 +277,7 @@ int st_batch_record(st_batch *b, int64_t capacity);
 int st_batch_record_fetch(st_batch *b, int64_t *segments, uint64_t *keys, double *sums,
                           int64_t *count);
 void st_batch_destroy(st_batch *b);
+
 
 /* ---- synthetic track library (deterministic, Philox-driven) ---- */
 typedef struct {

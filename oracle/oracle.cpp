@@ -113,6 +113,19 @@ int32_t orc_bin(const double *edges, int32_t nbins, double v)
     return (int32_t)(ub - edges) - 1;
 }
 
+// Tolerance bookkeeping for comparing a histogram filled from approximate
+// sums with one filled from these sums: v may move to a neighbouring bin only
+// if an edge adjacent to it lies within rtol * |v| (the edges at positions
+// lower_bound(v) - 1 and lower_bound(v), as numpy's searchsorted 'left').
+int32_t orc_near_edge(const double *edges, int32_t nbins, double v, double rtol)
+{
+    if (!(v == v)) return 0;
+    const int64_t i = std::lower_bound(edges, edges + nbins + 1, v) - edges;
+    for (int64_t j = i - 1; j <= i; ++j)
+        if (j >= 0 && j <= nbins && std::fabs(v - edges[j]) <= rtol * std::fabs(v)) return 1;
+    return 0;
+}
+
 // VoxelConstrainedSphereMethod.cu:138-154.
 void orc_track_zshift(double *z, int64_t n, double half_length)
 {
@@ -200,8 +213,9 @@ int64_t orc_score_oversample(const double *x, const double *y, const double *zp,
     score_pairs(x, y, zp, e, n, *g, sx, sy, sz, pk, py);
     const int64_t m = (int64_t)pk.size();
     if (n_pairs) *n_pairs = m;
-    if (pair_keys) std::memcpy(pair_keys, pk.data(), m * sizeof(uint64_t));
-    if (pair_y) std::memcpy(pair_y, py.data(), m * sizeof(double));
+    // memcpy from an empty vector's data() (possibly null) is undefined even for 0 bytes
+    if (pair_keys && m) std::memcpy(pair_keys, pk.data(), m * sizeof(uint64_t));
+    if (pair_y && m) std::memcpy(pair_y, py.data(), m * sizeof(double));
     return reduce_canonical(pk, py, scratch, keys, sums);
 }
 
@@ -221,7 +235,8 @@ double orc_run(const double *x, const double *y, const double *z, const double *
                int32_t shift_xy, int32_t shift_z,
                const double *edges0, int32_t nbins0, uint64_t *bins0,
                const double *edges1, int32_t nbins1, uint64_t *bins1,
-               int32_t n_threads, int32_t *threads_used)
+               int32_t n_threads, int32_t *threads_used,
+               double near_rtol, uint64_t *near_counts)
 {
     if (track_stride < 1) track_stride = 1;
     std::vector<int64_t> tracks;
@@ -236,6 +251,7 @@ double orc_run(const double *x, const double *y, const double *z, const double *
     if (threads_used) *threads_used = used;
     std::vector<std::vector<uint64_t>> priv0(used, std::vector<uint64_t>(bins0 ? nbins0 : 0, 0));
     std::vector<std::vector<uint64_t>> priv1(used, std::vector<uint64_t>(bins1 ? nbins1 : 0, 0));
+    std::vector<uint64_t> near_priv(2 * (size_t)used, 0);
 
     auto t0 = std::chrono::steady_clock::now();
 #pragma omp parallel
@@ -273,10 +289,12 @@ double orc_run(const double *x, const double *y, const double *z, const double *
                 if (bins0) {
                     int32_t bi = orc_bin(edges0, nbins0, sums[i]);
                     if (bi >= 0) priv0[tid][bi]++;
+                    if (near_counts && orc_near_edge(edges0, nbins0, sums[i], near_rtol)) near_priv[2 * tid]++;
                 }
                 if (bins1) {
                     int32_t bi = orc_bin(edges1, nbins1, sums[i]);
                     if (bi >= 0) priv1[tid][bi]++;
+                    if (near_counts && orc_near_edge(edges1, nbins1, sums[i], near_rtol)) near_priv[2 * tid + 1]++;
                 }
             }
         }
@@ -285,6 +303,10 @@ double orc_run(const double *x, const double *y, const double *z, const double *
     for (int i = 0; i < used; ++i) {
         if (bins0) for (int32_t j = 0; j < nbins0; ++j) bins0[j] += priv0[i][j];
         if (bins1) for (int32_t j = 0; j < nbins1; ++j) bins1[j] += priv1[i][j];
+        if (near_counts) {
+            near_counts[0] += near_priv[2 * i];
+            near_counts[1] += near_priv[2 * i + 1];
+        }
     }
     return std::chrono::duration<double>(t1 - t0).count();
 }

@@ -70,6 +70,12 @@ void orc_edges(int32_t is_log, float lo, float hi, int32_t nbins, double *edges)
  * otherwise -1 (NaN is never counted). */
 int32_t orc_bin(const double *edges, int32_t nbins, double v);
 
+/* 1 if an edge adjacent to v (positions lower_bound(v) - 1 and lower_bound(v))
+ * lies within rtol * |v| of it, else 0 (NaN: 0).  Not a reference function:
+ * the allowance when a histogram filled from sums with relative error below
+ * rtol is compared with one filled from the oracle's sums. */
+int32_t orc_near_edge(const double *edges, int32_t nbins, double v, double rtol);
+
 /* ShiftTrackZOrigin (VoxelConstrainedSphereMethod.cu:138-154): z[i] += H'. */
 void orc_track_zshift(double *z, int64_t n, double half_length);
 
@@ -99,8 +105,10 @@ int64_t orc_reduce_pairs(const uint64_t *pair_keys, const double *pair_y, int64_
  * geoms[geom_of_track[t]] (geom_of_track may be NULL: geometry 0) and shift
  * counter track_index = track_index_base + t.  Only tracks with
  * t % track_stride == 0 are processed.  Up to two histograms (lin and/or log)
- * are filled from the same sums.  Returns the wall seconds of the scoring loop
- * and writes the thread count used to *threads_used. */
+ * are filled from the same sums.  If near_counts is not NULL, near_counts[h]
+ * is increased by the number of sums for which orc_near_edge(histogram h,
+ * near_rtol) holds.  Returns the wall seconds of the scoring loop and writes
+ * the thread count used to *threads_used. */
 double orc_run(const double *x, const double *y, const double *z, const double *e,
                const int64_t *offsets, int64_t n_tracks, int64_t track_stride,
                const orc_geometry *geoms, const int32_t *geom_of_track,
@@ -108,7 +116,8 @@ double orc_run(const double *x, const double *y, const double *z, const double *
                int32_t shift_xy, int32_t shift_z,
                const double *edges0, int32_t nbins0, uint64_t *bins0,
                const double *edges1, int32_t nbins1, uint64_t *bins1,
-               int32_t n_threads, int32_t *threads_used);
+               int32_t n_threads, int32_t *threads_used,
+               double near_rtol, uint64_t *near_counts);
 
 #ifdef __cplusplus
 }

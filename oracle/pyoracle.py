@@ -13,7 +13,7 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+LIB_PATH = os.environ.get("SUPERTRACK_ORACLE_LIBRARY") or os.path.join(HERE, "liboracle.so")
 PHILOX_CHECK = os.path.join(HERE, "philox_rocrand_check")
 
 _d = C.POINTER(C.c_double)
@@ -64,7 +64,9 @@ def lib():
     L.orc_reduce_pairs.restype = C.c_int64
     L.orc_run.argtypes = [_d, _d, _d, _d, _i64, C.c_int64, C.c_int64, C.POINTER(OrcGeometry), _i32,
                           C.c_int32, C.c_uint64, C.c_uint32, C.c_int32, C.c_int32, _d, C.c_int32, _u64,
-                          _d, C.c_int32, _u64, C.c_int32, _i32]
+                          _d, C.c_int32, _u64, C.c_int32, _i32, C.c_double, _u64]
+    L.orc_near_edge.argtypes = [_d, C.c_int32, C.c_double, C.c_double]
+    L.orc_near_edge.restype = C.c_int32
     L.orc_run.restype = C.c_double
     _lib = L
     return L
@@ -156,9 +158,16 @@ def track_segments(x, y, z, e, g: OrcGeometry, seed: int, track_index: int, n_os
     return [score_oversample(x, y, zp, e, g, *sh[k]) for k in range(n_os)]
 
 
+def near_edge(e: np.ndarray, v: float, rtol: float) -> bool:
+    """True if an edge adjacent to v lies within rtol * |v| of it."""
+    e = np.ascontiguousarray(e, np.float64)
+    return bool(lib().orc_near_edge(_p(e, _d), len(e) - 1, v, rtol))
+
+
 def run(x, y, z, e, offsets, geoms, geom_of_track, n_os, seed, shift_xy=True, shift_z=False,
-        hists=(), track_stride=1, track_index_base=0, n_threads=0):
-    """Whole-run driver: returns ([bins per hist], seconds, threads)."""
+        hists=(), track_stride=1, track_index_base=0, n_threads=0, near_rtol=None):
+    """Whole-run driver: returns ([bins per hist], seconds, threads), and with
+    near_rtol set also [near-edge sums per hist] (orc_near_edge)."""
     x = np.ascontiguousarray(x, np.float64)
     y = np.ascontiguousarray(y, np.float64)
     z = np.ascontiguousarray(z, np.float64)
@@ -170,14 +179,19 @@ def run(x, y, z, e, offsets, geoms, geom_of_track, n_os, seed, shift_xy=True, sh
     bins = [None if h is None else np.zeros(len(h) - 1, np.uint64) for h in hs]
     ed = [None if h is None else np.ascontiguousarray(h, np.float64) for h in hs]
     used = C.c_int32(0)
+    near = np.zeros(2, np.uint64)
     secs = lib().orc_run(
         _p(x, _d), _p(y, _d), _p(z, _d), _p(e, _d), _p(offsets, _i64), len(offsets) - 1, track_stride, garr,
         None if gt is None else _p(gt, _i32), n_os, seed, track_index_base, int(shift_xy), int(shift_z),
         None if ed[0] is None else _p(ed[0], _d), 0 if ed[0] is None else len(ed[0]) - 1,
         None if bins[0] is None else _p(bins[0], _u64),
         None if ed[1] is None else _p(ed[1], _d), 0 if ed[1] is None else len(ed[1]) - 1,
-        None if bins[1] is None else _p(bins[1], _u64), n_threads, C.byref(used))
-    return [b for b in bins if b is not None], float(secs), int(used.value)
+        None if bins[1] is None else _p(bins[1], _u64), n_threads, C.byref(used),
+        0.0 if near_rtol is None else float(near_rtol), None if near_rtol is None else _p(near, _u64))
+    out = [b for b in bins if b is not None]
+    if near_rtol is None:
+        return out, float(secs), int(used.value)
+    return out, float(secs), int(used.value), [int(near[h]) for h in range(len(out))]
 
 
 def rocrand_philox(rows):

@@ -17,7 +17,9 @@ fallback: without the library every entry point raises.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
+import weakref
 from typing import Iterable, Optional, Sequence
 
 import numpy as np
@@ -30,6 +32,46 @@ __all__ = [
     "Histogram", "TrackLibrary", "BatchRun", "synth_library", "run_plain", "ST_F32", "ST_F64",
     "ST_HIST_LIN", "ST_HIST_LOG", "SuperTrackError", "device_count", "synchronize", "default_stream_handle",
 ]
+
+
+# Wrappers that own library objects, freed in a fixed order at exit (before
+# the HIP runtime is torn down); after shutdown() the finalizers do nothing.
+_live = weakref.WeakSet()
+_closed = False
+
+
+def shutdown() -> None:
+    """Frees every live object (batches, libraries, histograms, methods,
+    pairs, tracks, device arrays, in that order), then calls st_shutdown,
+    which waits for the library's streams and destroys them.  Registered to
+    run at interpreter exit; objects must not be used afterwards."""
+    global _closed
+    if _closed:
+        return
+    objs = list(_live)
+    for cls in (BatchRun, TrackLibrary, Histogram, VoxelConstrainedSphereMethod, VolumeEdepPair, Track,
+                DeviceArray):
+        for o in objs:
+            if isinstance(o, cls):
+                o.free()
+    if _capi._lib is not None:
+        check = _capi._lib.st_shutdown()
+        if check != 0:
+            raise SuperTrackError(check, "st_shutdown", _capi._lib.st_last_error().decode())
+    _closed = True
+
+
+atexit.register(shutdown)
+
+
+def _finalize(obj) -> None:
+    """__del__ body: frees the object unless the library has been shut down."""
+    if _closed:
+        return
+    try:
+        obj.free()
+    except Exception:
+        pass
 
 
 def _dptr(a: np.ndarray, dtype) -> C.POINTER:
@@ -80,6 +122,7 @@ class DeviceArray:
         self.nbytes = int(nbytes)
         self.ptr = C.c_void_p()
         call("st_malloc", C.byref(self.ptr), C.c_size_t(max(self.nbytes, 1)))
+        _live.add(self)
 
     @classmethod
     def from_numpy(cls, a: np.ndarray) -> "DeviceArray":
@@ -101,10 +144,7 @@ class DeviceArray:
             self.ptr = C.c_void_p()
 
     def __del__(self):
-        try:
-            self.free()
-        except Exception:
-            pass
+        _finalize(self)
 
 
 class Track:
@@ -112,6 +152,7 @@ class Track:
 
     def __init__(self, x=None, y=None, z=None, edep=None, capacity: int = 0):
         self.c = _capi.Track()
+        _live.add(self)
         if x is not None:
             self.upload(x, y, z, edep)
         else:
@@ -153,10 +194,7 @@ class Track:
             lib().st_track_free(C.byref(self.c))
 
     def __del__(self):
-        try:
-            self.free()
-        except Exception:
-            pass
+        _finalize(self)
 
 
 class VolumeEdepPair:
@@ -165,15 +203,17 @@ class VolumeEdepPair:
     def __init__(self, capacity: int):
         self.c = _capi.Pairs()
         call("st_pairs_alloc", C.byref(self.c), C.c_int64(capacity))
+        _live.add(self)
 
     @property
     def capacity(self) -> int:
         return int(self.c.capacity)
 
     def count(self) -> int:
-        v = np.zeros(1, np.int32)
-        call("st_memcpy_dtoh", v.ctypes.data_as(C.c_void_p), C.c_void_p(self.c.num_elements), C.c_size_t(4))
-        return int(v[0])
+        """*numElements, read on the host after the device has finished (as the
+        reference reads its managed counter after the track's synchronisation)."""
+        synchronize()
+        return int(C.cast(self.c.num_elements, C.POINTER(C.c_int32))[0])
 
     def download(self):
         m = self.count()
@@ -194,18 +234,14 @@ class VolumeEdepPair:
         if m:
             call("st_memcpy_htod", C.c_void_p(self.c.volume), keys.ctypes.data_as(C.c_void_p), C.c_size_t(8 * m))
             call("st_memcpy_htod", C.c_void_p(self.c.edep), ys.ctypes.data_as(C.c_void_p), C.c_size_t(8 * m))
-        cnt = np.array([m], np.int32)
-        call("st_memcpy_htod", C.c_void_p(self.c.num_elements), cnt.ctypes.data_as(C.c_void_p), C.c_size_t(4))
+        call("st_pairs_set_count", C.byref(self.c), C.c_int32(m))
 
     def free(self):
         if self.c.volume:
             lib().st_pairs_free(C.byref(self.c))
 
     def __del__(self):
-        try:
-            self.free()
-        except Exception:
-            pass
+        _finalize(self)
 
 
 class VoxelConstrainedSphereMethod:
@@ -217,6 +253,7 @@ class VoxelConstrainedSphereMethod:
         self.h = C.c_void_p()
         call("st_vcs_create", C.byref(geometry.c), int(bool(shift_track)), int(bool(shift_track_z)),
              int(suggested_blocks), int(suggested_threads), C.byref(self.h))
+        _live.add(self)
 
     def allocate_track_process(self, track: Track, seed: int, track_index: int, n_oversamples: int):
         call("st_vcs_begin_track", self.h, C.byref(track.c), C.c_uint64(seed), C.c_uint32(track_index),
@@ -232,16 +269,18 @@ class VoxelConstrainedSphereMethod:
         """Runs this method's work on a given hipStream_t (one per worker)."""
         call("st_vcs_set_stream", self.h, stream)
 
+    def set_queued(self, on: bool = True):
+        """Queued oversamples: a track's oversamples run as one launch when the
+        track ends (st_vcs_set_queued); the pairs are defined only after that."""
+        call("st_vcs_set_queued", self.h, int(bool(on)))
+
     def free(self):
         if self.h:
             lib().st_vcs_destroy(self.h)
             self.h = C.c_void_p()
 
     def __del__(self):
-        try:
-            self.free()
-        except Exception:
-            pass
+        _finalize(self)
 
 
 class Histogram:
@@ -257,6 +296,7 @@ class Histogram:
         self.kind = kind
         self.h = C.c_void_p()
         call("st_hist_create", kinds[kind], C.c_float(lo), C.c_float(hi), C.c_int32(nbins), C.byref(self.h))
+        _live.add(self)
         self.nbins = int(nbins)
         self.edges = np.empty(self.nbins + 1, np.float64)
         call("st_hist_edges", self.h, self.edges.ctypes.data_as(_capi.c_double_p))
@@ -290,10 +330,7 @@ class Histogram:
             self.h = C.c_void_p()
 
     def __del__(self):
-        try:
-            self.free()
-        except Exception:
-            pass
+        _finalize(self)
 
 
 class TrackLibrary:
@@ -315,6 +352,7 @@ class TrackLibrary:
         self.h = C.c_void_p()
         call("st_library_create", px, py, pz, pe, po, C.c_int64(n_tracks), garr, len(geometries), pg,
              int(precision), C.byref(self.h))
+        _live.add(self)
         self.geometries = list(geometries)
 
     @classmethod
@@ -329,6 +367,7 @@ class TrackLibrary:
         garr = (_capi.Geometry * len(geometries))(*[g.c for g in geometries])
         call("st_library_begin", C.c_int64(n_tracks), C.c_int64(n_points), garr, len(geometries), int(precision),
              C.byref(self.h))
+        _live.add(self)
         for x, y, z, e, offsets, gt in chunks:
             xs, px = _dptr(x, np.float64)
             ys, py = _dptr(y, np.float64)
@@ -354,10 +393,7 @@ class TrackLibrary:
             self.h = C.c_void_p()
 
     def __del__(self):
-        try:
-            self.free()
-        except Exception:
-            pass
+        _finalize(self)
 
 
 class BatchRun:
@@ -384,6 +420,7 @@ class BatchRun:
         self.library = library
         self.histograms = list(histograms)
         call("st_batch_create", library.h, C.byref(o), hs, len(histograms), C.byref(self.h))
+        _live.add(self)
 
     def run(self):
         call("st_batch_run", self.h)
@@ -402,6 +439,14 @@ class BatchRun:
         c = np.zeros(3, np.int64)
         call("st_batch_stats", self.h, c.ctypes.data_as(_capi.c_int64_p))
         return {"points_scored": int(c[0]), "fp64_rechecks": int(c[1]), "points_in_sphere": int(c[2])}
+
+    def balance(self):
+        """Conservation totals of the last run (modulo 2^64): energy words into
+        and out of the sphere tables, entries created and retired."""
+        c = np.zeros(4, np.uint64)
+        call("st_batch_balance", self.h, c.ctypes.data_as(_capi.c_uint64_p))
+        return {"energy_in": int(c[0]), "energy_out": int(c[1]), "entries_created": int(c[2]),
+                "entries_retired": int(c[3])}
 
     def params(self):
         p = np.zeros(4, np.int32)
@@ -428,10 +473,7 @@ class BatchRun:
             self.h = C.c_void_p()
 
     def __del__(self):
-        try:
-            self.free()
-        except Exception:
-            pass
+        _finalize(self)
 
 
 def synth_counts(spec: dict, n_tracks: int, t0: int = 0) -> np.ndarray:
@@ -470,12 +512,16 @@ def synth_library(spec: dict, n_tracks: int, t0: int = 0):
 
 def run_plain(x, y, z, e, offsets, geometry: SphericalGeometry, histogram: Histogram, n_oversamples: int,
               seed: int, shift_xy: bool = True, shift_z: bool = False, track_index_base: int = 0,
-              tracks: Optional[Iterable[int]] = None):
+              tracks: Optional[Iterable[int]] = None, queued: bool = False):
     """Per-track loop of SuperTrackManager.cu:58-83 over host arrays (the plain drop-in path).
 
-    Counts are added to histogram.cpu.  Returns the number of tracks processed.
+    Counts are added to histogram.cpu.  queued: the method queues each track's
+    oversamples and runs them in one launch at the track's end.  Returns the
+    number of tracks processed.
     """
     method = VoxelConstrainedSphereMethod(geometry, shift_xy, shift_z)
+    if queued:
+        method.set_queued(True)
     n_tracks = len(offsets) - 1
     done = 0
     for t in (range(n_tracks) if tracks is None else tracks):

@@ -9,7 +9,9 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libsupertrack_amd.so")
+# SUPERTRACK_AMD_LIBRARY: another build of the same library (e.g. with host
+# sanitizers); it is loaded the same way and there is still no fallback
+LIB_PATH = os.environ.get("SUPERTRACK_AMD_LIBRARY") or os.path.join(HERE, "libsupertrack_amd.so")
 
 ST_OK = 0
 ST_HIST_LIN = 0
@@ -35,7 +37,7 @@ class Track(C.Structure):
 
 class Pairs(C.Structure):
     _fields_ = [("volume", C.c_void_p), ("edep", C.c_void_p), ("num_elements", C.c_void_p),
-                ("capacity", C.c_int64), ("key_bound", C.c_uint64)]
+                ("capacity", C.c_int64), ("key_bound", C.c_uint64), ("count_dev", C.c_void_p)]
 
 
 class LibraryInfo(C.Structure):
@@ -75,12 +77,15 @@ PROTOTYPES = {
     "st_memset": (C.c_int, [C.c_void_p, C.c_int, C.c_size_t]),
     "st_device_synchronize": (C.c_int, []),
     "st_default_stream": (C.c_int, [C.POINTER(C.c_void_p)]),
+    "st_shutdown": (C.c_int, []),
+    "st_debug_fail_allocation": (C.c_int, [C.c_int64]),
     "st_geometry_make": (C.c_int, [C.c_double, C.c_double, C.POINTER(Geometry)]),
     "st_track_alloc": (C.c_int, [C.POINTER(Track), C.c_int64]),
     "st_track_upload": (C.c_int, [C.POINTER(Track), c_double_p, c_double_p, c_double_p, c_double_p, C.c_int64]),
     "st_track_free": (None, [C.POINTER(Track)]),
     "st_pairs_alloc": (C.c_int, [C.POINTER(Pairs), C.c_int64]),
     "st_pairs_free": (None, [C.POINTER(Pairs)]),
+    "st_pairs_set_count": (C.c_int, [C.POINTER(Pairs), C.c_int32]),
     "st_vcs_create": (C.c_int, [C.POINTER(Geometry), C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                 C.POINTER(C.c_void_p)]),
     "st_vcs_set_stream": (C.c_int, [C.c_void_p, C.c_void_p]),
@@ -88,6 +93,7 @@ PROTOTYPES = {
     "st_vcs_process_oversample": (C.c_int, [C.c_void_p, C.POINTER(Track), C.POINTER(Pairs)]),
     "st_vcs_end_track": (C.c_int, [C.c_void_p]),
     "st_vcs_destroy": (None, [C.c_void_p]),
+    "st_vcs_set_queued": (C.c_int, [C.c_void_p, C.c_int32]),
     "st_hist_create": (C.c_int, [C.c_int32, C.c_float, C.c_float, C.c_int32, C.POINTER(C.c_void_p)]),
     "st_hist_set_stream": (C.c_int, [C.c_void_p, C.c_void_p]),
     "st_hist_nbins": (C.c_int, [C.c_void_p, c_int32_p]),
@@ -115,6 +121,7 @@ PROTOTYPES = {
     "st_batch_params": (C.c_int, [C.c_void_p, c_int32_p]),
     "st_batch_times": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
     "st_batch_stats": (C.c_int, [C.c_void_p, c_int64_p]),
+    "st_batch_balance": (C.c_int, [C.c_void_p, c_uint64_p]),
     "st_batch_record": (C.c_int, [C.c_void_p, C.c_int64]),
     "st_batch_record_fetch": (C.c_int, [C.c_void_p, c_int64_p, c_uint64_p, c_double_p, c_int64_p]),
     "st_batch_destroy": (None, [C.c_void_p]),

@@ -41,18 +41,15 @@
 #include "st_common.hpp"
 #include "st_internal.hpp"
 #include "st_device.hpp"
+#include "layout.hpp"
 
 namespace st {
 namespace {
 
 constexpr int kBT = 256;                    // threads per workgroup
-constexpr int kBlk = 256;                   // points per library block
 constexpr unsigned long long kEmpty = ~0ull;
 constexpr int kMaxGeoms = 16;
 constexpr int kMaxHists = 2;
-constexpr int64_t kMaxWindow = 768;         // fast-path bound on spheres in a 2-layer window
-constexpr int64_t kMaxFastPoints = (int64_t)1 << 22;
-constexpr uint32_t kFlagExact = 1u;
 
 constexpr double kTwo24 = 16777216.0;
 constexpr double kTwo40 = 1099511627776.0;
@@ -60,23 +57,7 @@ constexpr double kTwo32 = 4294967296.0;
 constexpr double kTwoM24 = 1.0 / 16777216.0;
 constexpr double kTwoM64 = 1.0 / 18446744073709551616.0;
 constexpr double kTwoM32 = 1.0 / 4294967296.0;
-constexpr double kTwoM44 = 1.0 / 17592186044416.0;
 
-// Per-point lineal energy limits of the fused kernel (the fixed-point
-// conversions below are exact only under 2^51): ST_F32 y * 2^32, ST_F64
-// y * 2^24.
-constexpr double kMaxPointY32 = 262144.0;        // 2^18
-constexpr double kMaxPointY64 = 67108864.0;      // 2^26
-
-struct TrackDesc {
-    int64_t offset;     // first point in the sorted arrays
-    int64_t block0;     // first block in the block arrays
-    int32_t n;
-    int32_t nblocks;
-    int32_t geom;
-    uint32_t flags;
-    double xmin, xmax, ymin, ymax, zmax;
-};
 
 struct GeomDev {
     double H, L, D, invD, r2, gso, denom, denom_inv;
@@ -89,22 +70,6 @@ struct GeomDev {
     unsigned long long N, N2;
 };
 
-// ST_F32 block record (48 B, read through the scalar cache).  The fast path
-// works in lattice units: q = (coordinate - gso) / D, so rint(q) is the
-// sphere index and q - rint(q) the offset from the sphere centre; the voxel
-// faces are the half-way planes q = -1/2 and q = N - 1/2, so a point that is
-// not within dl of a half-way plane is inside the voxel iff every rint(q)
-// lies in [0, N - 1].
-//   oq*     (origin - gso) / D of the block's FP32 offsets (z with H' added);
-//   dl      bound on |approximate - reference| of any shifted, offset
-//           coordinate of the block in nm (FP32 rounding of the offsets plus,
-//           with a wide margin, FP64 rounding of every step, any |shift| <= H');
-//   fm      |q - rint(q)| >= fm: within dl of a half-way plane;
-//   mrgq    |sum (rint(q) - q)^2 - r^2 / D^2| <= mrgq: within the error of
-//           the squared distance.
-struct BlockF32 {
-    double oqx, oqy, oqz, dl, fm, mrgq;
-};
 
 struct HistDev {
     const double *edges;
@@ -112,6 +77,24 @@ struct HistDev {
     BinGuess guess;
     int lds_offset;     // in u32 units from the histogram LDS base
 };
+
+// Per-pass control block in device memory, cleared by batch_reset_kernel
+// before every pass (a kernel, not a memset: the pair is captured in a graph).
+//   stats[0] points scored, [1] exact FP64 rechecks, [2] in-sphere points,
+//   [3] fixed-point energy inserted into the tables, [4] fixed-point energy of
+//   the entries retired (binned or outside every histogram), [5] table entries
+//   created, [6] entries retired.  [3]/[4] are sums modulo 2^64 (of the ST_F64
+//   hi word plus the lo word times an odd constant); a pass that loses no
+//   entry has [3] == [4] and [5] == [6].
+constexpr int kStats = 7;
+struct BatchCtl {
+    unsigned long long ticket;
+    unsigned long long rec_count;
+    int error;   // 1: a table filled up, 2: a flush found more entries than its list bound
+    int pad;
+    unsigned long long stats[kStats];
+};
+constexpr unsigned long long kBalanceMul = 0x9E3779B97F4A7C15ull;
 
 struct BatchArgs {
     // library
@@ -128,9 +111,7 @@ struct BatchArgs {
     int32_t shift_xy, shift_z;
     uint32_t track_index_base;
     uint64_t seed;
-    unsigned long long *ticket;
-    int *error;
-    unsigned long long *stats;   // [3]: points scored, exact FP64 rechecks, in-sphere points
+    BatchCtl *ctl;
     // histograms
     HistDev hist[kMaxHists];
     int n_hists;
@@ -138,7 +119,6 @@ struct BatchArgs {
 
     // recording (test hook)
     int64_t rec_cap;
-    unsigned long long *rec_count;
     int64_t *rec_seg;
     unsigned long long *rec_key;
     double *rec_sum;
@@ -236,6 +216,11 @@ struct Acc<ST_F64> {
     {
         return (double)(long long)a0 * kTwoM24 + (double)(long long)a1 * kTwoM64;
     }
+    // conservation word of an entry or a point (see BatchCtl)
+    __device__ static unsigned long long balance(unsigned long long a0, unsigned long long a1)
+    {
+        return a0 + a1 * kBalanceMul;
+    }
 };
 
 template <>
@@ -245,6 +230,7 @@ struct Acc<ST_F32> {
     {
         return (double)(long long)a0 * kTwoM32;
     }
+    __device__ static unsigned long long balance(unsigned long long a0, unsigned long long) { return a0; }
 };
 
 // Home slot in [0, SLOTS) of a 64-bit key: 32-bit multiplicative hash
@@ -340,6 +326,10 @@ void batch_kernel(BatchArgs a)
     const int lane = tid & 63;
     // per-thread statistics, summed at exit
     unsigned int n_scored = 0, n_flagged = 0, n_in = 0;
+    // conservation: energy words into and out of the table, entries created
+    // and retired (a lost entry breaks one of the two equalities)
+    unsigned long long bal_in = 0, bal_out = 0;
+    unsigned int n_created = 0, n_retired = 0;
     for (int s = tid; s < SLOTS; s += kBT) {
         tkey[s] = kEmpty;
         tacc0[s] = 0;
@@ -387,7 +377,7 @@ void batch_kernel(BatchArgs a)
         __syncthreads();
         int n = uniform_i32(s_occ);
         if (n > kLP * kBT) {   // cannot happen (the bound above); reported, never read past the list
-            if (tid == 0) atomicOr(a.error, 2);
+            if (tid == 0) atomicOr(&a.ctl->error, 2);
             n = kLP * kBT < SLOTS ? kLP * kBT : SLOTS;
         }
         unsigned long long kk[kLP], k0[kLP], k1[kLP];
@@ -416,8 +406,12 @@ void batch_kernel(BatchArgs a)
                     k0[r] = v0;
                     k1[r] = v1;
                     vv[u] = Acc<MODE>::value(v0, v1);
+                    if (em[u]) {
+                        bal_out += Acc<MODE>::balance(v0, v1);
+                        ++n_retired;
+                    }
                     if (em[u] && a.rec_cap > 0) {
-                        const unsigned long long p = atomicAdd(a.rec_count, 1ull);
+                        const unsigned long long p = atomicAdd(&a.ctl->rec_count, 1ull);
                         if ((int64_t)p < a.rec_cap) {
                             // test hook only: the segment id from the ticket
                             const long long tk = s_seg;
@@ -487,7 +481,7 @@ void batch_kernel(BatchArgs a)
         // segment set-up (ticket, shift, block range: barriers) at raised
         // priority as well, back to normal for the point loop
         __builtin_amdgcn_s_setprio(2);
-        if (tid == 0) s_seg = (long long)atomicAdd(a.ticket, 1ull);
+        if (tid == 0) s_seg = (long long)atomicAdd(&a.ctl->ticket, 1ull);
         __syncthreads();
         const int64_t seg = uniform_i64(s_seg);
         __syncthreads();
@@ -626,11 +620,15 @@ void batch_kernel(BatchArgs a)
                         prev = atomicCAS(&tkey[h], kEmpty, key);
                     }
                     if (prev == kEmpty || prev == key) {
-                        if (prev == kEmpty) fresh[j] = (int)h;
+                        if (prev == kEmpty) {
+                            fresh[j] = (int)h;
+                            ++n_created;
+                        }
                         atomicAdd(&tacc0[h], (unsigned long long)v0);
                         if (kWords == 2) atomicAdd(&tacc1[h], (unsigned long long)v1);
+                        bal_in += Acc<MODE>::balance((unsigned long long)v0, (unsigned long long)v1);
                     } else {
-                        atomicOr(a.error, 1);
+                        atomicOr(&a.ctl->error, 1);
                     }
                 }
             }
@@ -675,17 +673,15 @@ void batch_kernel(BatchArgs a)
         }
     }
     {
-        unsigned long long c0 = n_scored, c1 = n_flagged, c2 = n_in;
+        unsigned long long c[kStats] = {n_scored, n_flagged, n_in, bal_in, bal_out, n_created, n_retired};
 #pragma unroll
         for (int d = 32; d >= 1; d >>= 1) {
-            c0 += __shfl_xor(c0, d, 64);
-            c1 += __shfl_xor(c1, d, 64);
-            c2 += __shfl_xor(c2, d, 64);
+#pragma unroll
+            for (int k = 0; k < kStats; ++k) c[k] += __shfl_xor(c[k], d, 64);
         }
         if (lane == 0) {
-            atomicAdd(&a.stats[0], c0);
-            atomicAdd(&a.stats[1], c1);
-            atomicAdd(&a.stats[2], c2);
+#pragma unroll
+            for (int k = 0; k < kStats; ++k) atomicAdd(&a.ctl->stats[k], c[k]);
         }
     }
     __syncthreads();
@@ -701,15 +697,15 @@ void batch_kernel(BatchArgs a)
     }
 }
 
-__global__ void batch_reset_kernel(unsigned long long *ticket, int *error, unsigned long long *rec_count,
-                                   unsigned long long *stats)
+__global__ void batch_reset_kernel(BatchCtl *ctl)
 {
     if (threadIdx.x == 0) {
-        *ticket = 0;
-        *error = 0;
-        if (rec_count) *rec_count = 0;
-        stats[0] = stats[1] = stats[2] = 0;
+        ctl->ticket = 0;
+        ctl->rec_count = 0;
+        ctl->error = 0;
+        ctl->pad = 0;
     }
+    if (threadIdx.x < kStats) ctl->stats[threadIdx.x] = 0;
 }
 
 template <int MODE, int SLOTS, int PPT>
@@ -744,7 +740,7 @@ int dev_alloc(T **dst, size_t n, int64_t *bytes)
 {
     *dst = nullptr;
     if (n == 0) return ST_OK;
-    hipError_t e = hipMalloc((void **)dst, n * sizeof(T));
+    hipError_t e = dev_malloc((void **)dst, n * sizeof(T));
     if (e != hipSuccess) {
         *dst = nullptr;
         return set_error(ST_ENOMEM, "st_library_begin: hipMalloc(%zu B) failed: %s", n * sizeof(T),
@@ -767,7 +763,7 @@ int dev_upload(T **dst, const T *src, size_t n, int64_t *bytes)
 {
     *dst = nullptr;
     if (n == 0) return ST_OK;
-    hipError_t e = hipMalloc((void **)dst, n * sizeof(T));
+    hipError_t e = dev_malloc((void **)dst, n * sizeof(T));
     if (e != hipSuccess) {
         *dst = nullptr;
         return set_error(ST_ENOMEM, "st_library_create: hipMalloc(%zu B) failed: %s", n * sizeof(T),
@@ -778,39 +774,13 @@ int dev_upload(T **dst, const T *src, size_t n, int64_t *bytes)
     return ST_OK;
 }
 
-// Largest number of distinct spheres that the points of any z-window of
-// width w can touch, for any shift: min(points, x-cells * y-cells * layers).
-int64_t window_bound(const double *x, const double *y, const double *z, int64_t n, double D, double w,
-                     int64_t layers)
-{
-    if (n == 0) return 0;
-    std::deque<int64_t> xmax, xmin, ymax, ymin;
-    int64_t best = 0;
-    int64_t j = 0;
-    for (int64_t i = 0; i < n; ++i) {
-        while (j < n && z[j] - z[i] <= w) {
-            while (!xmax.empty() && x[xmax.back()] <= x[j]) xmax.pop_back();
-            xmax.push_back(j);
-            while (!xmin.empty() && x[xmin.back()] >= x[j]) xmin.pop_back();
-            xmin.push_back(j);
-            while (!ymax.empty() && y[ymax.back()] <= y[j]) ymax.pop_back();
-            ymax.push_back(j);
-            while (!ymin.empty() && y[ymin.back()] >= y[j]) ymin.pop_back();
-            ymin.push_back(j);
-            ++j;
-        }
-        const int64_t cnt = j - i;
-        const double cx = std::floor((x[xmax.front()] - x[xmin.front()]) / D) + 2.0;
-        const double cy = std::floor((y[ymax.front()] - y[ymin.front()]) / D) + 2.0;
-        const double cells = cx * cy * (double)layers;
-        const int64_t b = cells < (double)cnt ? (int64_t)cells : cnt;
-        best = std::max(best, b);
-        if (best >= kMaxWindow + 1) return best;   // already ineligible
-        for (auto *dq : {&xmax, &xmin, &ymax, &ymin})
-            while (!dq->empty() && dq->front() <= i) dq->pop_front();
-    }
-    return best;
-}
+
+// Launch shape of the fused kernel.
+struct Launch {
+    int slots = 1536, blocks_per_cu = 1, ppt = 1, grid = 0;
+    bool lds_hist = true;     // histograms privatised in LDS (else device atomics)
+    size_t lds = 0;
+};
 
 }  // namespace
 }  // namespace st
@@ -831,6 +801,7 @@ struct st_library {
     BlockF32 *blocks = nullptr;
     TrackDesc *tracks_dev = nullptr;
     GeomDev *geoms_dev = nullptr;
+    int32_t *order_dev = nullptr;      // fast_order on the device
     // streamed construction: begin sizes the device arrays, appends fill them
     // in track order, finish publishes the track table
     int64_t tracks_done = 0, points_done = 0, blocks_cap = 0;
@@ -842,15 +813,9 @@ struct st_batch {
     st_batch_opts opts{};
     std::vector<st_hist *> hists;
     hipStream_t stream = nullptr;
-    int threads = kBT, slots = 2048, blocks_per_cu = 4, grid = 0, ppt = 1;
+    Launch ln;
     int hist_words = 0;
-    bool lds_hist = true;     // histograms privatised in LDS (else device atomics)
-    size_t lds = 0;
-    int32_t *order_dev = nullptr;
-    unsigned long long *ticket = nullptr;
-    int *error = nullptr;
-    unsigned long long *rec_count = nullptr;
-    unsigned long long *stats = nullptr;   // [3] per run, see BatchArgs
+    BatchCtl *ctl = nullptr;   // per-pass control block (device)
     int64_t rec_cap = 0;
     int64_t *rec_seg = nullptr;
     unsigned long long *rec_key = nullptr;
@@ -868,9 +833,9 @@ struct st_batch {
 
 namespace {
 
-BatchArgs make_args(st_batch *b)
+BatchArgs make_args(const st_library *L, const st_batch_opts &o, st_hist *const *hists, int n_hists,
+                    const Launch &ln, BatchCtl *ctl)
 {
-    st_library *L = b->lib;
     BatchArgs a{};
     a.tracks = L->tracks_dev;
     a.x64 = L->x64;
@@ -881,66 +846,113 @@ BatchArgs make_args(st_batch *b)
     a.zfirst = L->zfirst;
     a.blocks = L->blocks;
     a.geoms = L->geoms_dev;
-    a.order = b->order_dev;
-    a.n_seg = (int64_t)L->fast_order.size() * b->opts.n_oversamples;
-    a.n_os = std::max(b->opts.n_oversamples, 1);
-    a.shift_xy = b->opts.shift_xy;
-    a.shift_z = b->opts.shift_z;
-    a.track_index_base = b->opts.track_index_base;
-    a.seed = b->opts.seed;
-    a.ticket = b->ticket;
-    a.error = b->error;
-    a.n_hists = (int)b->hists.size();
+    a.order = L->order_dev;
+    a.n_seg = (int64_t)L->fast_order.size() * o.n_oversamples;
+    a.n_os = std::max(o.n_oversamples, 1);
+    a.shift_xy = o.shift_xy;
+    a.shift_z = o.shift_z;
+    a.track_index_base = o.track_index_base;
+    a.seed = o.seed;
+    a.ctl = ctl;
+    a.n_hists = n_hists;
     int off = 0;
-    for (int h = 0; h < a.n_hists; ++h) {
-        a.hist[h].edges = b->hists[h]->edges_dev;
-        a.hist[h].bins = (unsigned long long *)b->hists[h]->bins_dev;
-        a.hist[h].guess = b->hists[h]->guess;
+    for (int h = 0; h < n_hists; ++h) {
+        a.hist[h].edges = hists[h]->edges_dev;
+        a.hist[h].bins = (unsigned long long *)hists[h]->bins_dev;
+        a.hist[h].guess = hists[h]->guess;
         a.hist[h].lds_offset = off;
-        off += b->hists[h]->nbins;
+        off += hists[h]->nbins;
     }
-    a.hist_lds_words = b->lds_hist ? off : 0;
-    a.rec_cap = b->rec_cap;
-    a.rec_count = b->rec_count;
-    a.stats = b->stats;
-    a.rec_seg = b->rec_seg;
-    a.rec_key = b->rec_key;
-    a.rec_sum = b->rec_sum;
+    a.hist_lds_words = ln.lds_hist ? off : 0;
     return a;
 }
 
-int launch_fast(st_batch *b, int slots, int grid)
+// Queues one pass: the control-block reset, then the fused kernel.
+int launch_pass(int precision, const BatchArgs &args, const Launch &ln, hipStream_t stream)
 {
-    BatchArgs a = make_args(b);
-    batch_reset_kernel<<<1, 64, 0, b->stream>>>(b->ticket, b->error, b->rec_cap > 0 ? b->rec_count : nullptr,
-                                                 b->stats);
-    if (a.n_seg == 0) return ST_OK;
-    const size_t lds = lds_bytes(b->lib->precision, slots, b->lds_hist ? b->hist_words : 0);
-    const void *k = pick_kernel(b->lib->precision, slots, b->ppt);
-    void *args[] = {&a};
-    ST_HIP_TRY(hipLaunchKernel(k, dim3(grid), dim3(kBT), args, lds, b->stream));
+    batch_reset_kernel<<<1, 64, 0, stream>>>(args.ctl);
+    ST_HIP_TRY(hipGetLastError());
+    if (args.n_seg == 0) return ST_OK;
+    const void *k = pick_kernel(precision, ln.slots, ln.ppt);
+    BatchArgs a = args;
+    void *kargs[] = {&a};
+    ST_HIP_TRY(hipLaunchKernel(k, dim3(ln.grid), dim3(kBT), kargs, ln.lds, stream));
     return ST_OK;
 }
 
-int configure(st_batch *b, int slots, int bpc, int ppt)
+int launch_fast(st_batch *b)
 {
-    b->ppt = ppt == 2 ? 2 : 1;
+    BatchArgs a = make_args(b->lib, b->opts, b->hists.data(), (int)b->hists.size(), b->ln, b->ctl);
+    a.rec_cap = b->rec_cap;
+    a.rec_seg = b->rec_seg;
+    a.rec_key = b->rec_key;
+    a.rec_sum = b->rec_sum;
+    return launch_pass(b->lib->precision, a, b->ln, b->stream);
+}
+
+// Launch shape for (table slots, workgroups per CU, points per thread),
+// capped by what the kernel's registers and LDS allow.
+int configure_launch(int precision, int hist_words, int slots, int bpc, int ppt, Launch *out)
+{
+    Launch ln;
+    ln.ppt = ppt == 2 ? 2 : 1;
     // privatise the histograms in LDS when they fit next to the table at two
     // workgroups per CU; otherwise count straight into the device bins
-    b->lds_hist = lds_bytes(b->lib->precision, slots, b->hist_words) <= 80 * 1024;
-    const size_t lds = lds_bytes(b->lib->precision, slots, b->lds_hist ? b->hist_words : 0);
+    ln.lds_hist = lds_bytes(precision, slots, hist_words) <= 80 * 1024;
+    const size_t lds = lds_bytes(precision, slots, ln.lds_hist ? hist_words : 0);
     if (lds > 160 * 1024)
         return set_error(ST_ERANGE, "batched kernel needs %zu B of LDS (limit 160 KiB)", lds);
-    const void *k = pick_kernel(b->lib->precision, slots, b->ppt);
+    const void *k = pick_kernel(precision, slots, ln.ppt);
     if (!k) return set_error(ST_EINVAL, "batched kernel: %d table slots take one point per thread only", slots);
     ST_HIP_TRY(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     int per_cu = 0;
     ST_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, kBT, lds));
     if (per_cu < 1) return set_error(ST_EHIP, "batched kernel cannot be resident with %zu B LDS", lds);
-    b->slots = slots;
-    b->blocks_per_cu = std::max(1, std::min(bpc, per_cu));
-    b->grid = b->blocks_per_cu * device_cu_count();
-    b->lds = lds;
+    ln.slots = slots;
+    ln.blocks_per_cu = std::max(1, std::min(bpc, per_cu));
+    ln.grid = ln.blocks_per_cu * device_cu_count();
+    ln.lds = lds;
+    *out = ln;
+    return ST_OK;
+}
+
+int configure(st_batch *b, int slots, int bpc, int ppt)
+{
+    return configure_launch(b->lib->precision, b->hist_words, slots, bpc, ppt, &b->ln);
+}
+
+// The launch shape used unless the options name one or ask for tuning: every
+// table size holds the largest survivor set (kMaxWindow) or a quarter table,
+// plus the points of the iterations between occupancy checks, so every
+// fast-path track fits.  Defaults: the fastest shapes measured on MI355X for
+// 2 nm and 10 nm spheres.
+int default_launch(int precision, int hist_words, const st_batch_opts &o, Launch *out)
+{
+    const bool f32 = precision == ST_F32;
+    const int slots = o.table_slots ? o.table_slots : 1536;
+    if (slots != 1280 && slots != 1536 && slots != 2048 && slots != 4096)
+        return set_error(ST_EINVAL, "batched run: table_slots must be 1280, 1536, 2048 or 4096");
+    const int bpc = o.blocks_per_cu ? o.blocks_per_cu : (f32 ? 5 : 3);
+    const int ppt = o.points_per_thread ? o.points_per_thread : (f32 ? 1 : 2);
+    if (ppt != 1 && ppt != 2) return set_error(ST_EINVAL, "batched run: points_per_thread must be 1 or 2");
+    int rc = configure_launch(precision, hist_words, slots, bpc, ppt, out);
+    if (rc) return rc;
+    // FP32 with fewer than five workgroups per CU (large privatised
+    // histograms): a 1280-slot table if it fits more workgroups, else two
+    // points per thread (both measured faster on MI355X in that case)
+    if (f32 && !o.points_per_thread && out->blocks_per_cu < 5) {
+        const int fit1536 = out->blocks_per_cu;
+        if (!o.table_slots) {
+            Launch alt;
+            if (configure_launch(precision, hist_words, 1280, bpc, 1, &alt) == ST_OK &&
+                alt.blocks_per_cu > fit1536) {
+                *out = alt;
+                return ST_OK;
+            }
+            clear_error();
+        }
+        return configure_launch(precision, hist_words, slots, bpc, 2, out);
+    }
     return ST_OK;
 }
 
@@ -983,6 +995,83 @@ int run_exact(st_batch *b)
         }
         m->in_track = false;
     }
+    return ST_OK;
+}
+
+// Times one full pass of every launch candidate and keeps the fastest.  The
+// histograms' counters are saved before and restored after, so tuning adds
+// no counts; the library's streams are non-blocking, so every stream that may
+// still write those counters is drained first.
+int autotune(st_batch *b)
+{
+    struct Events {
+        hipEvent_t e[2] = {nullptr, nullptr};
+        ~Events()
+        {
+            for (hipEvent_t x : e)
+                if (x) (void)hipEventDestroy(x);
+        }
+    } ev;
+    ST_HIP_TRY(hipEventCreate(&ev.e[0]));
+    ST_HIP_TRY(hipEventCreate(&ev.e[1]));
+    ST_HIP_TRY(hipStreamSynchronize(b->stream));
+    std::vector<std::vector<uint64_t>> saved;
+    for (st_hist *h : b->hists) {
+        ST_HIP_TRY(hipStreamSynchronize(h->stream));
+        saved.emplace_back(h->nbins);
+        ST_HIP_TRY(hipMemcpyAsync(saved.back().data(), h->bins_dev, h->nbins * sizeof(uint64_t),
+                                  hipMemcpyDeviceToHost, b->stream));
+    }
+    ST_HIP_TRY(hipStreamSynchronize(b->stream));
+    const int cand_slots[] = {1280, 1536, 2048, 4096};
+    const int cand_bpc[] = {2, 3, 4, 5, 6, 8};
+    float best = 1e30f;
+    int best_s = b->ln.slots, best_b = b->ln.blocks_per_cu, best_p = b->ln.ppt;
+    for (int s : cand_slots)
+        for (int pp = 1; pp <= 2; ++pp)
+            for (int bp : cand_bpc) {
+                if (configure(b, s, bp, pp) != ST_OK) continue;
+                if (b->ln.blocks_per_cu != bp) continue;
+                ST_HIP_TRY(hipEventRecord(ev.e[0], b->stream));
+                int rc = launch_fast(b);
+                if (rc) return rc;
+                ST_HIP_TRY(hipEventRecord(ev.e[1], b->stream));
+                ST_HIP_TRY(hipEventSynchronize(ev.e[1]));
+                float ms = 0;
+                ST_HIP_TRY(hipEventElapsedTime(&ms, ev.e[0], ev.e[1]));
+                if (ms < best) {
+                    best = ms;
+                    best_s = s;
+                    best_b = bp;
+                    best_p = pp;
+                }
+            }
+    for (size_t i = 0; i < b->hists.size(); ++i)
+        ST_HIP_TRY(hipMemcpyAsync(b->hists[i]->bins_dev, saved[i].data(), saved[i].size() * sizeof(uint64_t),
+                                  hipMemcpyHostToDevice, b->stream));
+    ST_HIP_TRY(hipStreamSynchronize(b->stream));
+    clear_error();   // candidates that did not fit set an error string
+    return configure(b, best_s, best_b, best_p);
+}
+
+// Waits for the work queued on `stream` and copies the control block back.
+int read_ctl(const BatchCtl *ctl, hipStream_t stream, BatchCtl *out)
+{
+    ST_HIP_TRY(hipMemcpyAsync(out, ctl, sizeof *out, hipMemcpyDeviceToHost, stream));
+    ST_HIP_TRY(hipStreamSynchronize(stream));
+    return ST_OK;
+}
+
+// The kernel's error flag and the conservation totals of a finished pass.
+int check_ctl(const BatchCtl &c)
+{
+    if (c.error)
+        return set_error(ST_ESTATE, "batched kernel reported a full sphere table or entry list (code %d)", c.error);
+    if (c.stats[3] != c.stats[4] || c.stats[5] != c.stats[6])
+        return set_error(ST_ESTATE,
+                         "batched kernel lost sphere-table entries: energy words in %016llx / out %016llx, "
+                         "entries created %llu / retired %llu",
+                         c.stats[3], c.stats[4], c.stats[5], c.stats[6]);
     return ST_OK;
 }
 
@@ -1061,133 +1150,14 @@ int st_library_append(st_library *L, const double *x, const double *y, const dou
         bbase = last.block0 + last.nblocks;
     }
     TrackDesc *tds = L->tracks.data() + L->tracks_done;
-    int64_t nblocks = 0;
-    for (int64_t t = 0; t < n_tracks; ++t) {
-        TrackDesc &td = tds[t];
-        td.offset = pbase + (offsets[t] - offsets[0]);
-        td.n = (int32_t)(offsets[t + 1] - offsets[t]);
-        td.nblocks = (td.n + kBlk - 1) / kBlk;
-        td.block0 = bbase + nblocks;
-        td.geom = geom_of_track ? geom_of_track[t] : 0;
-        td.flags = 0;
-        nblocks += td.nblocks;
-    }
+    const int64_t nblocks = plan_chunk(offsets, n_tracks, geom_of_track, pbase, bbase, tds);
     if (bbase + nblocks > L->blocks_cap)
         return set_error(ST_EINVAL, "st_library_append: block table overflow");
-
-    // host staging for this chunk, indexed from the chunk's first point/block
-    std::vector<double> sx(n_points), sy(n_points), sz(n_points), se(n_points);
-    std::vector<double> zfirst(std::max<int64_t>(nblocks, 1));
-    std::vector<float4> p32;
-    std::vector<BlockF32> blocks;
-    if (precision == ST_F32) {
-        p32.resize(n_points);
-        blocks.resize(std::max<int64_t>(nblocks, 1));
-    }
-    const double *X = x, *Y = y, *Z = z, *E = edep;
-    const int64_t base = offsets[0];
-    parallel_for(n_tracks, [&](int64_t t) {
-        TrackDesc &td = tds[t];
-        const int64_t o = td.offset - pbase, n = td.n;
-        const int64_t b0 = td.block0 - bbase;
-        std::vector<int64_t> idx(n);
-        std::iota(idx.begin(), idx.end(), 0);
-        const double *zt = Z + base + o;
-        auto zkey = [&](int64_t i) { return std::isnan(zt[i]) ? HUGE_VAL : zt[i]; };
-        std::stable_sort(idx.begin(), idx.end(), [&](int64_t a2, int64_t b2) { return zkey(a2) < zkey(b2); });
-        bool finite = true;
-        double xmin = HUGE_VAL, xmax = -HUGE_VAL, ymin = HUGE_VAL, ymax = -HUGE_VAL, esum = 0.0, emax = 0.0;
-        for (int64_t k = 0; k < n; ++k) {
-            const int64_t s = base + o + idx[k];
-            sx[o + k] = X[s];
-            sy[o + k] = Y[s];
-            sz[o + k] = Z[s];
-            se[o + k] = E[s];
-            finite = finite && std::isfinite(X[s]) && std::isfinite(Y[s]) && std::isfinite(Z[s]) &&
-                     std::isfinite(E[s]);
-            xmin = std::min(xmin, X[s]);
-            xmax = std::max(xmax, X[s]);
-            ymin = std::min(ymin, Y[s]);
-            ymax = std::max(ymax, Y[s]);
-            esum += std::fabs(E[s]);
-            emax = std::max(emax, std::fabs(E[s]));
-        }
-        td.xmin = xmin;
-        td.xmax = xmax;
-        td.ymin = ymin;
-        td.ymax = ymax;
-        td.zmax = n ? sz[o + n - 1] : 0.0;
-        for (int b = 0; b < td.nblocks; ++b) zfirst[b0 + b] = sz[o + (int64_t)b * kBlk];
-        const st_geometry &g = L->geoms[td.geom];
-        const double denom = (2. / 3.) * g.diameter;
-        // fixed-point headroom: sum of |y| over the track
-        const double ysum = esum / denom;
-        const double ylim = (precision == ST_F64) ? 2.7e11 : 1.0e9;
-        const double ypoint = (precision == ST_F64) ? kMaxPointY64 : kMaxPointY32;
-        bool fast = finite && n <= kMaxFastPoints && ysum < ylim && emax / denom < ypoint;
-        if (fast && n > 0) {
-            // the fast path reads sphere indices from the low words of the
-            // rounded lattice coordinates: |q| < 2^30 for every shift |s| <= H'
-            const double cmax = std::max({std::fabs(xmin), std::fabs(xmax), std::fabs(ymin), std::fabs(ymax),
-                                          std::fabs(sz[o] + g.half_length),
-                                          std::fabs(sz[o + n - 1] + g.half_length)});
-            fast = (cmax + g.half_length + std::fabs(g.gso)) / g.diameter < 1073741824.0;
-        }
-        if (fast) {
-            // entries kept by a flush come from the next point's layer (or, for
-            // keys with an index equal to N, the layer below): a z-window two
-            // diameters wide, touching at most three layers
-            const double w = 2.0 * g.diameter * (1.0 + 1e-12) + 1e-9;
-            const int64_t W = window_bound(sx.data() + o, sy.data() + o, sz.data() + o, n, g.diameter, w, 3);
-            fast = W <= kMaxWindow;
-        }
-        td.flags = fast ? 0u : kFlagExact;
-        if (precision == ST_F32) {
-            for (int b = 0; b < td.nblocks; ++b) {
-                const int64_t p0 = o + (int64_t)b * kBlk, p1 = std::min<int64_t>(p0 + kBlk, o + n);
-                double lo3[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL}, hi3[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
-                const double *arr[3] = {sx.data(), sy.data(), sz.data()};
-                for (int64_t p = p0; p < p1; ++p)
-                    for (int a3 = 0; a3 < 3; ++a3) {
-                        lo3[a3] = std::min(lo3[a3], arr[a3][p]);
-                        hi3[a3] = std::max(hi3[a3], arr[a3][p]);
-                    }
-                double org[3];
-                for (int a3 = 0; a3 < 3; ++a3) {
-                    org[a3] = (std::isfinite(lo3[a3]) && std::isfinite(hi3[a3])) ? 0.5 * (lo3[a3] + hi3[a3]) : 0.0;
-                }
-                double err = 0.0;
-                for (int64_t p = p0; p < p1; ++p) {
-                    for (int a3 = 0; a3 < 3; ++a3) {
-                        const double d = arr[a3][p] - org[a3];
-                        const float f = (float)d;
-                        (a3 == 0 ? p32[p].x : a3 == 1 ? p32[p].y : p32[p].z) = f;
-                        const double e1 = std::fabs((double)f - d) + std::fabs(d) * 2.3e-16;
-                        if (std::isfinite(e1)) err = std::max(err, e1);
-                        else err = HUGE_VAL;
-                    }
-                    p32[p].w = (float)se[p];
-                }
-                // dl: the FP32 rounding of the offsets plus, with a wide margin,
-                // the FP64 rounding of each add and product on the way to the
-                // sphere test (|shift| <= H' on each axis)
-                BlockF32 &bk = blocks[b0 + b];
-                const double invD = 1.0 / g.diameter;
-                bk.oqx = (org[0] - g.gso) * invD;
-                bk.oqy = (org[1] - g.gso) * invD;
-                bk.oqz = ((org[2] + g.half_length) - g.gso) * invD;
-                double dl = err + kTwoM44 * (std::fabs(org[0]) + std::fabs(org[1]) + std::fabs(org[2]) +
-                                             7.0 * g.half_length + std::fabs(g.gso) + g.diameter + 16.0);
-                if (!std::isfinite(dl)) dl = HUGE_VAL;
-                const double dlq = dl * invD;
-                const double r2q = (g.radius * g.radius) / (g.diameter * g.diameter);
-                bk.dl = dl;
-                bk.fm = 0.5 - (dlq + 1e-12);
-                bk.mrgq = 3.0 * dlq * (1.0 + 2.0 * dlq) + 1e-14 * r2q;
-            }
-        }
-    });
-
+    ChunkLayout lay;
+    fill_chunk(x, y, z, edep, offsets, n_tracks, L->geoms.data(), precision, pbase, bbase, nblocks, tds, lay);
+    std::vector<double> &sx = lay.x, &sy = lay.y, &sz = lay.z, &se = lay.e, &zfirst = lay.zfirst;
+    std::vector<PointF32> &p32 = lay.p32;
+    std::vector<BlockF32> &blocks = lay.blocks;
     int rc = ST_OK;
     if (!rc) rc = dev_put(L->x64 + pbase, sx.data(), n_points);
     if (!rc) rc = dev_put(L->y64 + pbase, sy.data(), n_points);
@@ -1195,7 +1165,7 @@ int st_library_append(st_library *L, const double *x, const double *y, const dou
     if (!rc) rc = dev_put(L->e64 + pbase, se.data(), n_points);
     if (!rc) rc = dev_put(L->zfirst + bbase, zfirst.data(), nblocks);
     if (!rc && precision == ST_F32) {
-        rc = dev_put(L->p32 + pbase, p32.data(), n_points);
+        rc = dev_put(reinterpret_cast<PointF32 *>(L->p32) + pbase, p32.data(), n_points);
         if (!rc) rc = dev_put(L->blocks + bbase, blocks.data(), nblocks);
     }
     if (rc) return rc;
@@ -1250,6 +1220,7 @@ int st_library_finish(st_library *L)
     int64_t bytes = 0;
     if (!rc) rc = dev_upload(&L->tracks_dev, L->tracks.data(), L->tracks.size(), &bytes);
     if (!rc) rc = dev_upload(&L->geoms_dev, gd.data(), gd.size(), &bytes);
+    if (!rc) rc = dev_upload(&L->order_dev, L->fast_order.data(), L->fast_order.size(), &bytes);
     L->device_bytes += bytes;
     if (rc) return rc;
     L->finished = true;
@@ -1305,7 +1276,7 @@ void st_library_destroy(st_library *L)
     if (!L) return;
     (void)hipDeviceSynchronize();
     void *ptrs[] = {L->x64, L->y64, L->z64, L->e64, L->p32,
-                    L->zfirst, L->blocks, L->tracks_dev, L->geoms_dev};
+                    L->zfirst, L->blocks, L->tracks_dev, L->geoms_dev, L->order_dev};
     for (void *p : ptrs) (void)hipFree(p);
     delete L;
 }
@@ -1334,18 +1305,9 @@ int st_batch_create(st_library *lib, const st_batch_opts *opts, st_hist *const *
         st_batch_destroy(b);
         return code;
     };
-    hipError_t e = hipMalloc((void **)&b->ticket, sizeof(unsigned long long));
-    if (e == hipSuccess) e = hipMalloc((void **)&b->error, sizeof(int));
-    if (e == hipSuccess) e = hipMalloc((void **)&b->rec_count, sizeof(unsigned long long));
-    if (e == hipSuccess) e = hipMalloc((void **)&b->stats, 3 * sizeof(unsigned long long));
-    if (e == hipSuccess) e = hipMemset(b->stats, 0, 3 * sizeof(unsigned long long));
-    if (e == hipSuccess && !lib->fast_order.empty())
-        e = hipMalloc((void **)&b->order_dev, lib->fast_order.size() * sizeof(int32_t));
-    if (e == hipSuccess && !lib->fast_order.empty())
-        e = hipMemcpy(b->order_dev, lib->fast_order.data(), lib->fast_order.size() * sizeof(int32_t),
-                      hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemset(b->rec_count, 0, sizeof(unsigned long long));
-    if (e != hipSuccess) return fail(set_error(ST_EHIP, "st_batch_create: %s", hipGetErrorString(e)));
+    hipError_t e = dev_malloc((void **)&b->ctl, sizeof(BatchCtl));
+    if (e == hipSuccess) e = hipMemset(b->ctl, 0, sizeof(BatchCtl));
+    if (e != hipSuccess) return fail(set_error(hip_status(e), "st_batch_create: %s", hipGetErrorString(e)));
     for (const st_geometry &g : lib->geoms) {
         st_vcs *m = nullptr;
         rc = st_vcs_create(&g, opts->shift_xy, opts->shift_z, 0, 0, &m);
@@ -1353,75 +1315,10 @@ int st_batch_create(st_library *lib, const st_batch_opts *opts, st_hist *const *
         st_vcs_set_stream(m, b->stream);
         b->methods.push_back(m);
     }
-    // every table size holds the largest survivor set (kMaxWindow) or a
-    // quarter table, plus the points of the iterations between occupancy
-    // checks, so every fast-path track fits.  Defaults: the fastest shapes
-    // measured on MI355X for 2 nm and 10 nm spheres.
-    const bool f32 = lib->precision == ST_F32;
-    int slots = opts->table_slots ? opts->table_slots : 1536;
-    if (slots != 1280 && slots != 1536 && slots != 2048 && slots != 4096)
-        return fail(set_error(ST_EINVAL, "st_batch_create: table_slots must be 1280, 1536, 2048 or 4096"));
-    int bpc = opts->blocks_per_cu ? opts->blocks_per_cu : (f32 ? 5 : 3);
-    const int ppt = opts->points_per_thread ? opts->points_per_thread : (f32 ? 1 : 2);
-    if (ppt != 1 && ppt != 2) return fail(set_error(ST_EINVAL, "st_batch_create: points_per_thread must be 1 or 2"));
-    rc = configure(b, slots, bpc, ppt);
+    rc = default_launch(lib->precision, b->hist_words, *opts, &b->ln);
     if (rc) return fail(rc);
-    // FP32 with fewer than five workgroups per CU (large privatised
-    // histograms): a 1280-slot table if it fits more workgroups, else two
-    // points per thread (both measured faster on MI355X in that case)
-    if (f32 && !opts->points_per_thread && b->blocks_per_cu < 5) {
-        const int fit1536 = b->blocks_per_cu;
-        bool done = false;
-        if (!opts->table_slots) {
-            rc = configure(b, 1280, bpc, 1);
-            done = rc == ST_OK && b->blocks_per_cu > fit1536;
-        }
-        if (!done) {
-            rc = configure(b, slots, bpc, 2);
-            if (rc) return fail(rc);
-        }
-    }
     if (opts->autotune && !lib->fast_order.empty() && opts->n_oversamples > 0) {
-        // time each candidate on one full pass; keep the fastest
-        const int cand_slots[] = {1280, 1536, 2048, 4096};
-        const int cand_bpc[] = {2, 3, 4, 5, 6, 8};
-        hipEvent_t e0, e1;
-        ST_HIP_TRY(hipEventCreate(&e0));
-        ST_HIP_TRY(hipEventCreate(&e1));
-        float best = 1e30f;
-        int best_s = slots, best_b = b->blocks_per_cu, best_p = b->ppt;
-        std::vector<std::vector<uint64_t>> saved;
-        for (st_hist *h : b->hists) {
-            saved.emplace_back(h->nbins);
-            ST_HIP_TRY(hipMemcpy(saved.back().data(), h->bins_dev, h->nbins * sizeof(uint64_t),
-                                 hipMemcpyDeviceToHost));
-        }
-        for (int s : cand_slots)
-        for (int pp = 1; pp <= 2; ++pp) {
-            for (int bp : cand_bpc) {
-                if (configure(b, s, bp, pp) != ST_OK) continue;
-                if (b->blocks_per_cu != bp) continue;
-                ST_HIP_TRY(hipEventRecord(e0, b->stream));
-                rc = launch_fast(b, b->slots, b->grid);
-                if (rc) return fail(rc);
-                ST_HIP_TRY(hipEventRecord(e1, b->stream));
-                ST_HIP_TRY(hipEventSynchronize(e1));
-                float ms = 0;
-                ST_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-                if (ms < best) {
-                    best = ms;
-                    best_s = s;
-                    best_b = bp;
-                    best_p = pp;
-                }
-            }
-        }
-        for (size_t i = 0; i < b->hists.size(); ++i)
-            ST_HIP_TRY(hipMemcpy(b->hists[i]->bins_dev, saved[i].data(), saved[i].size() * sizeof(uint64_t),
-                                 hipMemcpyHostToDevice));
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        rc = configure(b, best_s, best_b, best_p);
+        rc = autotune(b);
         if (rc) return fail(rc);
     }
     *out = b;
@@ -1455,7 +1352,7 @@ int st_batch_run(st_batch *b)
     if (b->opts.use_graph && b->rec_cap == 0) {
         if (!b->graph_ready) {
             ST_HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-            int rc = launch_fast(b, b->slots, b->grid);
+            int rc = launch_fast(b);
             hipGraph_t g = nullptr;
             hipError_t e = hipStreamEndCapture(b->stream, &g);
             if (rc) return rc;
@@ -1466,7 +1363,7 @@ int st_batch_run(st_batch *b)
         }
         ST_HIP_TRY(hipGraphLaunch(b->exec, b->stream));
     } else {
-        int rc = launch_fast(b, b->slots, b->grid);
+        int rc = launch_fast(b);
         if (rc) return rc;
     }
     ST_HIP_TRY(hipEventRecord(b->ev[1], b->stream));
@@ -1489,30 +1386,39 @@ int st_batch_times(st_batch *b, float *ms2)
 int st_batch_sync(st_batch *b)
 {
     if (!b) return set_error(ST_EINVAL, "st_batch_sync: null batch");
-    ST_HIP_TRY(hipStreamSynchronize(b->stream));
-    int err = 0;
-    ST_HIP_TRY(hipMemcpy(&err, b->error, sizeof err, hipMemcpyDeviceToHost));
-    if (err) return set_error(ST_ESTATE, "batched kernel reported a full sphere table or entry list (code %d)", err);
-    return ST_OK;
+    BatchCtl c;
+    int rc = read_ctl(b->ctl, b->stream, &c);
+    if (rc) return rc;
+    return check_ctl(c);
 }
 
 int st_batch_stats(st_batch *b, int64_t *stats3)
 {
     if (!b || !stats3) return set_error(ST_EINVAL, "st_batch_stats: null argument");
-    ST_HIP_TRY(hipStreamSynchronize(b->stream));
-    unsigned long long c[3];
-    ST_HIP_TRY(hipMemcpy(c, b->stats, sizeof c, hipMemcpyDeviceToHost));
-    for (int i = 0; i < 3; ++i) stats3[i] = (int64_t)c[i];
+    BatchCtl c;
+    int rc = read_ctl(b->ctl, b->stream, &c);
+    if (rc) return rc;
+    for (int i = 0; i < 3; ++i) stats3[i] = (int64_t)c.stats[i];
+    return ST_OK;
+}
+
+int st_batch_balance(st_batch *b, uint64_t *balance4)
+{
+    if (!b || !balance4) return set_error(ST_EINVAL, "st_batch_balance: null argument");
+    BatchCtl c;
+    int rc = read_ctl(b->ctl, b->stream, &c);
+    if (rc) return rc;
+    for (int i = 0; i < 4; ++i) balance4[i] = c.stats[3 + i];
     return ST_OK;
 }
 
 int st_batch_params(const st_batch *b, int32_t *p)
 {
     if (!b || !p) return set_error(ST_EINVAL, "st_batch_params: null argument");
-    p[0] = b->threads;
-    p[1] = b->slots;
-    p[2] = b->blocks_per_cu;
-    p[3] = b->ppt;
+    p[0] = kBT;
+    p[1] = b->ln.slots;
+    p[2] = b->ln.blocks_per_cu;
+    p[3] = b->ln.ppt;
     return ST_OK;
 }
 
@@ -1527,9 +1433,9 @@ int st_batch_record(st_batch *b, int64_t capacity)
     b->rec_sum = nullptr;
     b->rec_cap = 0;
     if (capacity == 0) return ST_OK;
-    ST_HIP_TRY(hipMalloc((void **)&b->rec_seg, capacity * sizeof(int64_t)));
-    ST_HIP_TRY(hipMalloc((void **)&b->rec_key, capacity * sizeof(unsigned long long)));
-    ST_HIP_TRY(hipMalloc((void **)&b->rec_sum, capacity * sizeof(double)));
+    ST_ALLOC_TRY(dev_malloc((void **)&b->rec_seg, capacity * sizeof(int64_t)));
+    ST_ALLOC_TRY(dev_malloc((void **)&b->rec_key, capacity * sizeof(unsigned long long)));
+    ST_ALLOC_TRY(dev_malloc((void **)&b->rec_sum, capacity * sizeof(double)));
     b->rec_cap = capacity;
     return ST_OK;
 }
@@ -1537,9 +1443,10 @@ int st_batch_record(st_batch *b, int64_t capacity)
 int st_batch_record_fetch(st_batch *b, int64_t *segments, uint64_t *keys, double *sums, int64_t *count)
 {
     if (!b || !count) return set_error(ST_EINVAL, "st_batch_record_fetch: null argument");
-    ST_HIP_TRY(hipStreamSynchronize(b->stream));
-    unsigned long long c = 0;
-    ST_HIP_TRY(hipMemcpy(&c, b->rec_count, sizeof c, hipMemcpyDeviceToHost));
+    BatchCtl ctl;
+    int rc = read_ctl(b->ctl, b->stream, &ctl);
+    if (rc) return rc;
+    const unsigned long long c = ctl.rec_count;
     *count = (int64_t)c;
     if ((int64_t)c > b->rec_cap) return set_error(ST_ERANGE, "st_batch_record_fetch: %llu records, capacity %lld",
                                                  c, (long long)b->rec_cap);
@@ -1560,7 +1467,7 @@ void st_batch_destroy(st_batch *b)
         if (e) (void)hipEventDestroy(e);
     st_track_free(&b->scratch);
     st_pairs_free(&b->pairs);
-    void *ptrs[] = {b->order_dev, b->ticket, b->error, b->rec_count, b->stats, b->rec_seg, b->rec_key, b->rec_sum};
+    void *ptrs[] = {b->ctl, b->rec_seg, b->rec_key, b->rec_sum};
     for (void *p : ptrs) (void)hipFree(p);
     delete b;
 }

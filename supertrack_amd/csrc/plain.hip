@@ -10,7 +10,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
+#include <unordered_map>
 
 #include "st_common.hpp"
 #include "st_internal.hpp"
@@ -78,16 +80,51 @@ constexpr int kScoreRows = 4;
 constexpr int kScoreTile = kScoreBlock * kScoreRows;
 constexpr int kWaves = kScoreBlock / 64;
 
+// Geometry of the membership test, in the reference's units.
+struct ScoreGeom {
+    double H, D, invD, gso, r2;
+    uint64_t N;
+};
+
+// FilterInScoringBox (:183-198), FilterTrackInSphere (:249-271) and the key of
+// ScoreTrackInSphere (:313-318) for one point of a track whose z already
+// carries +H': true if the shifted point lies in the voxel and in its nearest
+// sphere, whose key is then written to key.
+__device__ __forceinline__ bool score_point(double x, double y, double z, double sx, double sy, double sz,
+                                            const ScoreGeom &g, uint64_t &key)
+{
+    const double xs = x + sx;
+    const double ys = y + sy;
+    const double zs = z + sz;
+    if (!(fabs(xs) < g.H && fabs(ys) < g.H && fabs(zs) < g.H)) return false;
+    const double ax = xs - g.gso;
+    const double ay = ys - g.gso;
+    const double az = zs - g.gso;
+    const double kx = nearest_index(ax, g.D, g.invD);
+    const double ky = nearest_index(ay, g.D, g.invD);
+    const double kz = nearest_index(az, g.D, g.invD);
+    const double dx = (kx * g.D) - ax;
+    const double dy = (ky * g.D) - ay;
+    const double dz = (kz * g.D) - az;
+    const double dist = ((dx * dx) + (dy * dy)) + (dz * dz);
+    if (!(dist <= g.r2)) return false;
+    const uint64_t ix = (uint64_t)(int64_t)kx;
+    const uint64_t iy = (uint64_t)(int64_t)ky;
+    const uint64_t iz = (uint64_t)(int64_t)kz;
+    key = ix + iy * g.N + iz * g.N * g.N;
+    return true;
+}
+
 struct ScoreArgs {
     const double *x, *y, *z, *lin;
     int64_t n;
-    double H, D, invD, gso, r2;
-    uint64_t N;
+    ScoreGeom g;
     const double *shifts;
     int os;
     uint64_t *out_key;
     double *out_y;
-    int32_t *out_count;
+    int32_t *out_count;        // device copy of the pair count (read by the histogram kernels)
+    int32_t *out_count_host;   // host-visible copy (VolumeEdepPair::numElements)
     unsigned long long *status;
     unsigned long long *ticket;
     unsigned long long ticket_base;
@@ -130,33 +167,9 @@ __global__ __launch_bounds__(kScoreBlock) void score_kernel(ScoreArgs a)
         key[r] = 0;
         yv[r] = 0.0;
         const int64_t i = base + (int64_t)r * kScoreBlock + tid;
-        if (i < a.n) {
-            // FilterInScoringBox (:183-198)
-            const double xs = a.x[i] + sx;
-            const double ys = a.y[i] + sy;
-            const double zs = a.z[i] + sz;
-            if (fabs(xs) < a.H && fabs(ys) < a.H && fabs(zs) < a.H) {
-                // FilterTrackInSphere (:249-271)
-                const double ax = xs - a.gso;
-                const double ay = ys - a.gso;
-                const double az = zs - a.gso;
-                const double kx = nearest_index(ax, a.D, a.invD);
-                const double ky = nearest_index(ay, a.D, a.invD);
-                const double kz = nearest_index(az, a.D, a.invD);
-                const double dx = (kx * a.D) - ax;
-                const double dy = (ky * a.D) - ay;
-                const double dz = (kz * a.D) - az;
-                const double dist = ((dx * dx) + (dy * dy)) + (dz * dz);
-                if (dist <= a.r2) {
-                    // ScoreTrackInSphere (:313-322)
-                    const uint64_t ix = (uint64_t)(int64_t)kx;
-                    const uint64_t iy = (uint64_t)(int64_t)ky;
-                    const uint64_t iz = (uint64_t)(int64_t)kz;
-                    key[r] = ix + iy * a.N + iz * a.N * a.N;
-                    yv[r] = a.lin[i];
-                    flag[r] = true;
-                }
-            }
+        if (i < a.n && score_point(a.x[i], a.y[i], a.z[i], sx, sy, sz, a.g, key[r])) {
+            yv[r] = a.lin[i];   // ScoreTrackInSphere's value (:322), computed once per track
+            flag[r] = true;
         }
     }
 
@@ -223,7 +236,10 @@ __global__ __launch_bounds__(kScoreBlock) void score_kernel(ScoreArgs a)
         }
         if (lane == 0) {
             s_excl = excl;
-            if (tile == a.num_tiles - 1) *a.out_count = (int32_t)(excl + tile_total);
+            if (tile == a.num_tiles - 1) {
+                *a.out_count = (int32_t)(excl + tile_total);
+                *a.out_count_host = (int32_t)(excl + tile_total);
+            }
         }
     }
     __syncthreads();
@@ -540,6 +556,92 @@ __global__ __launch_bounds__(kSortThreads) void hist_lds_kernel(HistArgs a)
 }
 
 // ---------------------------------------------------------------------------
+// queued oversamples: one workgroup per oversample, score -> sort -> bin
+// ---------------------------------------------------------------------------
+// A track of at most kLdsCap points whose oversamples were queued is run as one
+// launch: workgroup k scores every point for oversample os0 + k (tiles of
+// kSortThreads points, compacted in point order by wave ballots and a
+// running offset), writes its pairs to its own scratch row (the launch's last
+// oversample to the caller's pairs and counts, which the per-oversample path
+// would have left there), and sorts, reduces and bins them into every
+// histogram that consumed them, exactly as hist_lds_kernel does.
+constexpr int kMaxQueuedHists = 4;
+
+struct QueuedArgs {
+    const double *x, *y, *z, *lin;
+    int n;
+    ScoreGeom g;
+    const double *shifts;       // 3 per oversample of the track
+    int os0;                    // oversample of workgroup 0
+    int n_os;                   // workgroups
+    uint64_t *row_key;          // scratch rows of n pairs, one per workgroup
+    double *row_y;
+    int last_to_caller;         // the last workgroup writes to the caller's pairs
+    uint64_t *last_key;
+    double *last_y;
+    int32_t *last_count;        // device copy of the count
+    int32_t *last_count_host;   // host-visible copy (VolumeEdepPair::numElements)
+    HistArgs hist[kMaxQueuedHists];
+    int n_hists;
+};
+
+__global__ __launch_bounds__(kSortThreads) void queued_track_kernel(QueuedArgs a)
+{
+    extern __shared__ unsigned char smem[];
+    __shared__ uint32_t s_wcount[kSortWaves];
+    const int k = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const bool to_caller = a.last_to_caller && k == a.n_os - 1;
+    uint64_t *okey = to_caller ? a.last_key : a.row_key + (int64_t)k * a.n;
+    double *oy = to_caller ? a.last_y : a.row_y + (int64_t)k * a.n;
+    const int os = a.os0 + k;
+    const double sx = a.shifts[3 * os + 0];
+    const double sy = a.shifts[3 * os + 1];
+    const double sz = a.shifts[3 * os + 2];
+    const uint64_t lt_mask = (1ull << lane) - 1ull;
+    uint32_t base = 0;
+    for (int t0 = 0; t0 < a.n; t0 += kSortThreads) {
+        const int i = t0 + tid;
+        uint64_t key = 0;
+        const bool flag = i < a.n && score_point(a.x[i], a.y[i], a.z[i], sx, sy, sz, a.g, key);
+        const uint64_t m = __ballot(flag);
+        if (lane == 0) s_wcount[wave] = (uint32_t)__popcll(m);
+        __syncthreads();
+        uint32_t off = base, total = 0;
+#pragma unroll
+        for (int w = 0; w < kSortWaves; ++w) {
+            const uint32_t c = s_wcount[w];
+            off += (w < wave) ? c : 0u;
+            total += c;
+        }
+        if (flag) {
+            const uint32_t pos = off + (uint32_t)__popcll(m & lt_mask);
+            okey[pos] = key;
+            oy[pos] = a.lin[i];
+        }
+        base += total;
+        __syncthreads();   // s_wcount is rewritten by the next tile
+    }
+    const int m = (int)base;
+    if (to_caller && tid == 0) {
+        *a.last_count = m;
+        *a.last_count_host = m;
+    }
+    // the pairs written above are read back by other threads of the workgroup
+    __syncthreads();
+    if (m <= 0) return;
+    for (int h = 0; h < a.n_hists; ++h) {
+        HistArgs ha = a.hist[h];
+        ha.keys = okey;
+        ha.y = oy;
+        lds_sort_reduce_bin(ha, m, smem);
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Histogram, m > kLdsCap: onesweep LSD radix sort over global buffers
 // ---------------------------------------------------------------------------
 // os_digits_kernel  all passes' digit counts of all pairs (one read)
@@ -756,6 +858,18 @@ __global__ void zero_u64_kernel(unsigned long long *p, int64_t n)
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0;
 }
 
+ScoreGeom score_geom(const st_geometry &g)
+{
+    ScoreGeom s;
+    s.H = g.half_length;
+    s.D = g.diameter;
+    s.invD = 1.0 / g.diameter;
+    s.gso = g.gso;
+    s.r2 = g.radius * g.radius;
+    s.N = (uint64_t)g.n_linear;
+    return s;
+}
+
 inline int grid_for(int64_t n, int threads, int cap = 65535)
 {
     int64_t g = (n + threads - 1) / threads;
@@ -774,7 +888,7 @@ int grow(T **p, int64_t *cap, int64_t need)
         *cap = 0;
     }
     if (need <= 0) return ST_OK;
-    hipError_t e = hipMalloc((void **)p, (size_t)need * sizeof(T));
+    hipError_t e = dev_malloc((void **)p, (size_t)need * sizeof(T));
     if (e != hipSuccess) {
         *p = nullptr;
         return set_error(ST_ENOMEM, "hipMalloc(%lld x %zu B) failed: %s", (long long)need, sizeof(T),
@@ -785,24 +899,53 @@ int grow(T **p, int64_t *cap, int64_t need)
 }
 
 
+// The LDS kernels use more than 64 KiB of dynamic LDS: allowed once per device.
 int ensure_sort_attributes()
 {
-    static bool done = false;
-    if (done) return ST_OK;
+    constexpr int kDevices = 64;
+    static std::mutex mu;
+    static bool done[kDevices] = {};
+    int dev = 0;
+    ST_HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kDevices) return set_error(ST_ERANGE, "device %d beyond the %d supported", dev, kDevices);
+    std::lock_guard<std::mutex> lock(mu);
+    if (done[dev]) return ST_OK;
     ST_HIP_TRY(hipFuncSetAttribute((const void *)hist_lds_kernel,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBytes));
-    done = true;
+    ST_HIP_TRY(hipFuncSetAttribute((const void *)queued_track_kernel,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBytes));
+    done[dev] = true;
     return ST_OK;
 }
+
+// Pairs with queued oversamples pending, by the pairs' key array, and the
+// method that will produce them.
+std::mutex g_queue_mu;
+std::unordered_map<const void *, st_vcs *> g_queue;
+
+st_vcs *queue_producer(const void *pairs_key)
+{
+    std::lock_guard<std::mutex> lock(g_queue_mu);
+    auto it = g_queue.find(pairs_key);
+    return it == g_queue.end() ? nullptr : it->second;
+}
+
+// Largest scratch of queued pair rows per method (n points x oversamples x
+// 16 B); tracks that would need more run oversample by oversample.
+constexpr int64_t kQueueScratchBytes = (int64_t)1 << 30;
 
 // Onesweep buffers for `cap` pairs: ping-pong keys/payloads, look-back status
 // words per (tile, digit), the digit counts and the tile ticket.
 int ensure_sort_buffers(st_hist *h, int64_t cap)
 {
+    // each allocated on its own, so that a failure between them leaves no
+    // half-made state behind for the next call
     if (!h->digit_counts) {
-        ST_HIP_TRY(hipMalloc((void **)&h->digit_counts, (size_t)kMaxPasses * kGDigits * sizeof(unsigned int)));
+        ST_ALLOC_TRY(dev_malloc((void **)&h->digit_counts, (size_t)kMaxPasses * kGDigits * sizeof(unsigned int)));
         ST_HIP_TRY(hipMemset(h->digit_counts, 0, (size_t)kMaxPasses * kGDigits * sizeof(unsigned int)));
-        ST_HIP_TRY(hipMalloc((void **)&h->os_ticket, sizeof(unsigned long long)));
+    }
+    if (!h->os_ticket) {
+        ST_ALLOC_TRY(dev_malloc((void **)&h->os_ticket, sizeof(unsigned long long)));
         ST_HIP_TRY(hipMemset(h->os_ticket, 0, sizeof(unsigned long long)));
         h->os_ticket_base = 0;
     }
@@ -814,11 +957,11 @@ int ensure_sort_buffers(st_hist *h, int64_t cap)
         h->os_status = nullptr;
         h->sort_capacity = 0;
         const int64_t tiles = (cap + kOsTile - 1) / kOsTile;
-        ST_HIP_TRY(hipMalloc((void **)&h->keys_a, (size_t)cap * sizeof(uint64_t)));
-        ST_HIP_TRY(hipMalloc((void **)&h->keys_b, (size_t)cap * sizeof(uint64_t)));
-        ST_HIP_TRY(hipMalloc((void **)&h->vals_a, (size_t)cap * sizeof(uint32_t)));
-        ST_HIP_TRY(hipMalloc((void **)&h->vals_b, (size_t)cap * sizeof(uint32_t)));
-        ST_HIP_TRY(hipMalloc((void **)&h->os_status, (size_t)tiles * kGDigits * sizeof(unsigned long long)));
+        ST_ALLOC_TRY(dev_malloc((void **)&h->keys_a, (size_t)cap * sizeof(uint64_t)));
+        ST_ALLOC_TRY(dev_malloc((void **)&h->keys_b, (size_t)cap * sizeof(uint64_t)));
+        ST_ALLOC_TRY(dev_malloc((void **)&h->vals_a, (size_t)cap * sizeof(uint32_t)));
+        ST_ALLOC_TRY(dev_malloc((void **)&h->vals_b, (size_t)cap * sizeof(uint32_t)));
+        ST_ALLOC_TRY(dev_malloc((void **)&h->os_status, (size_t)tiles * kGDigits * sizeof(unsigned long long)));
         ST_HIP_TRY(hipMemset(h->os_status, 0, (size_t)tiles * kGDigits * sizeof(unsigned long long)));
         h->sort_capacity = cap;
     }
@@ -835,7 +978,7 @@ int hist_add_pairs(st_hist *h, const st_pairs *pairs, hipStream_t stream)
     HistArgs ha;
     ha.keys = pairs->volume;
     ha.y = pairs->edep;
-    ha.count = pairs->num_elements;
+    ha.count = pairs->count_dev;
     ha.edges = h->edges_dev;
     ha.guess = h->guess;
     ha.bins = (unsigned long long *)h->bins_dev;
@@ -850,7 +993,7 @@ int hist_add_pairs(st_hist *h, const st_pairs *pairs, hipStream_t stream)
     const int bits = 64 - __builtin_clzll(kb | 1ull);
     const int passes = (bits + kGBits - 1) / kGBits;
     const int tiles = (int)((pairs->capacity + kOsTile - 1) / kOsTile);
-    OsDigitsArgs da{pairs->volume, pairs->num_elements, h->digit_counts, passes};
+    OsDigitsArgs da{pairs->volume, pairs->count_dev, h->digit_counts, passes};
     os_digits_kernel<<<tiles, kOsThreads, 0, stream>>>(da);
     const uint64_t *src_k = nullptr;
     const uint32_t *src_v = nullptr;
@@ -863,7 +1006,7 @@ int hist_add_pairs(st_hist *h, const st_pairs *pairs, hipStream_t stream)
         pa.pair_keys = pairs->volume;
         pa.dst_k = bk[p & 1];
         pa.dst_v = bv[p & 1];
-        pa.count = pairs->num_elements;
+        pa.count = pairs->count_dev;
         pa.digit_counts = h->digit_counts + p * kGDigits;
         pa.status = h->os_status;
         pa.ticket = h->os_ticket;
@@ -880,6 +1023,100 @@ int hist_add_pairs(st_hist *h, const st_pairs *pairs, hipStream_t stream)
     os_reduce_kernel<<<grid_for(pairs->capacity, 256, 2048), 256, 0, stream>>>(ra);
     ST_HIP_TRY(hipGetLastError());
     return ST_OK;
+}
+
+int queue_flush(st_vcs *m)
+{
+    if (!m->queue_active) return ST_OK;
+    const int cnt = m->queue_count;
+    const int64_t n = m->n;
+    int rc = ensure_sort_attributes();
+    if (!rc) rc = grow(&m->queue_key, &m->queue_key_capacity, std::max<int64_t>((int64_t)cnt * n, 1));
+    if (!rc) rc = grow(&m->queue_y, &m->queue_y_capacity, std::max<int64_t>((int64_t)cnt * n, 1));
+    std::vector<st_hist *> consumers;
+    if (!rc) {
+        QueuedArgs a{};
+        a.x = m->queue_x;
+        a.y = m->queue_yc;
+        a.z = m->queue_z;
+        a.lin = m->lineal;
+        a.n = (int)n;
+        a.g = score_geom(m->g);
+        a.shifts = m->shifts;
+        a.last_key = m->queue_pairs.volume;
+        a.last_y = m->queue_pairs.edep;
+        a.last_count = m->queue_pairs.count_dev;
+        a.last_count_host = m->queue_pairs.num_elements;
+        // runs of consecutive oversamples consumed by the same histograms,
+        // up to kMaxQueuedHists histograms per launch
+        for (int k0 = 0; k0 < cnt && !rc;) {
+            int k1 = k0 + 1;
+            while (k1 < cnt && m->queue_consumers[k1] == m->queue_consumers[k0]) ++k1;
+            const std::vector<st_hist *> &cons = m->queue_consumers[k0];
+            const bool has_last = k1 == cnt;
+            for (size_t c0 = 0; c0 < std::max<size_t>(cons.size(), has_last ? 1 : 0) && !rc;
+                 c0 += kMaxQueuedHists) {
+                a.os0 = m->queue_os0 + k0;
+                a.n_os = k1 - k0;
+                a.row_key = m->queue_key + (int64_t)k0 * n;
+                a.row_y = m->queue_y + (int64_t)k0 * n;
+                a.last_to_caller = has_last && c0 == 0;
+                a.n_hists = 0;
+                for (size_t c = c0; c < cons.size() && c < c0 + kMaxQueuedHists; ++c) {
+                    st_hist *h = cons[c];
+                    HistArgs &ha = a.hist[a.n_hists++];
+                    ha.keys = nullptr;
+                    ha.y = nullptr;
+                    ha.count = nullptr;
+                    ha.edges = h->edges_dev;
+                    ha.guess = h->guess;
+                    ha.bins = (unsigned long long *)h->bins_dev;
+                    if (std::find(consumers.begin(), consumers.end(), h) == consumers.end()) consumers.push_back(h);
+                }
+                queued_track_kernel<<<a.n_os, kSortThreads, kLdsBytes, m->stream>>>(a);
+                if (hipGetLastError() != hipSuccess) rc = set_error(ST_EHIP, "queued_track_kernel launch failed");
+            }
+            k0 = k1;
+        }
+    }
+    // histograms on other streams wait for these launches
+    for (st_hist *h : consumers) {
+        if (rc || h->stream == m->stream) continue;
+        if (!m->queue_event && hipEventCreateWithFlags(&m->queue_event, hipEventDisableTiming) != hipSuccess) {
+            m->queue_event = nullptr;
+            rc = set_error(ST_EHIP, "queued oversamples: hipEventCreate failed");
+            break;
+        }
+        if (hipEventRecord(m->queue_event, m->stream) != hipSuccess ||
+            hipStreamWaitEvent(h->stream, m->queue_event, 0) != hipSuccess)
+            rc = set_error(ST_EHIP, "queued oversamples: stream ordering failed");
+    }
+    // the group ends whether or not it ran
+    {
+        std::lock_guard<std::mutex> lock(g_queue_mu);
+        auto it = g_queue.find(m->queue_pairs.volume);
+        if (it != g_queue.end() && it->second == m) g_queue.erase(it);
+    }
+    for (const auto &cons : m->queue_consumers)
+        for (st_hist *h : cons) h->producers.erase(std::remove(h->producers.begin(), h->producers.end(), m),
+                                                   h->producers.end());
+    m->queue_active = false;
+    m->queue_count = 0;
+    m->queue_consumers.clear();
+    return rc;
+}
+
+// Flushes every method whose queued oversamples h consumes.
+int queue_flush_producers(st_hist *h)
+{
+    int rc = ST_OK;
+    while (!h->producers.empty()) {
+        st_vcs *m = h->producers.back();
+        const int r = queue_flush(m);
+        if (r && !rc) rc = r;
+        h->producers.erase(std::remove(h->producers.begin(), h->producers.end(), m), h->producers.end());
+    }
+    return rc;
 }
 
 }  // namespace st
@@ -900,7 +1137,7 @@ int st_track_alloc(st_track *t, int64_t capacity)
     const size_t bytes = (size_t)(capacity + 1) * sizeof(double);
     double **arrs[4] = {&t->x, &t->y, &t->z, &t->edep};
     for (double **p : arrs) {
-        hipError_t e = hipMalloc((void **)p, bytes);
+        hipError_t e = dev_malloc((void **)p, bytes);
         if (e != hipSuccess) {
             st_track_free(t);
             return set_error(ST_ENOMEM, "st_track_alloc: hipMalloc(%zu) failed: %s", bytes,
@@ -947,24 +1184,37 @@ void st_track_free(st_track *t)
 }
 
 // VolumeEdepPair::Allocate (VolumeEdepPair.cu:6-17): numElements starts at
-// the capacity and the arrays start zeroed.
+// the capacity and the arrays start zeroed.  numElements is managed memory
+// in the reference, read on the host after the track's synchronisation
+// (Histogram.cu:105); here it is pinned, device-mapped host memory written by
+// the producer next to a device copy that the histogram kernels read.
 int st_pairs_alloc(st_pairs *p, int64_t capacity)
 {
     if (!p || capacity < 0 || capacity > 0x7FFFFFFF)
         return set_error(ST_EINVAL, "st_pairs_alloc: capacity must be in [0, 2^31)");
     std::memset(p, 0, sizeof *p);
     const size_t n = (size_t)(capacity > 0 ? capacity : 1);
-    hipError_t e1 = hipMalloc((void **)&p->volume, n * sizeof(uint64_t));
-    hipError_t e2 = hipMalloc((void **)&p->edep, n * sizeof(double));
-    hipError_t e3 = hipMalloc((void **)&p->num_elements, sizeof(int32_t));
-    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
+    hipError_t e1 = dev_malloc((void **)&p->volume, n * sizeof(uint64_t));
+    hipError_t e2 = dev_malloc((void **)&p->edep, n * sizeof(double));
+    hipError_t e3 = dev_malloc((void **)&p->count_dev, sizeof(int32_t));
+    hipError_t e4 = pinned_malloc((void **)&p->num_elements, sizeof(int32_t),
+                                  hipHostMallocMapped | hipHostMallocCoherent);
+    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess || e4 != hipSuccess) {
+        if (e4 != hipSuccess) p->num_elements = nullptr;
         st_pairs_free(p);
-        return set_error(ST_ENOMEM, "st_pairs_alloc: hipMalloc failed");
+        return set_error(ST_ENOMEM, "st_pairs_alloc: device or pinned host allocation failed");
+    }
+    // kernels store through the same address (unified addressing)
+    void *mapped = nullptr;
+    if (hipHostGetDevicePointer(&mapped, p->num_elements, 0) != hipSuccess || mapped != p->num_elements) {
+        st_pairs_free(p);
+        return set_error(ST_EHIP, "st_pairs_alloc: pinned count is not mapped at its host address");
     }
     ST_HIP_TRY(hipMemset(p->volume, 0, n * sizeof(uint64_t)));
     ST_HIP_TRY(hipMemset(p->edep, 0, n * sizeof(double)));
     const int32_t c = (int32_t)capacity;
-    ST_HIP_TRY(hipMemcpy(p->num_elements, &c, sizeof c, hipMemcpyHostToDevice));
+    ST_HIP_TRY(hipMemcpy(p->count_dev, &c, sizeof c, hipMemcpyHostToDevice));
+    *p->num_elements = c;
     p->capacity = capacity;
     p->key_bound = 0;
     return ST_OK;
@@ -973,13 +1223,30 @@ int st_pairs_alloc(st_pairs *p, int64_t capacity)
 void st_pairs_free(st_pairs *p)
 {
     if (!p) return;
+    if (p->volume) {
+        st_vcs *m = queue_producer(p->volume);
+        if (m) (void)queue_flush(m);
+    }
+    if (p->volume || p->edep || p->count_dev || p->num_elements) (void)hipDeviceSynchronize();
     (void)hipFree(p->volume);
     (void)hipFree(p->edep);
-    (void)hipFree(p->num_elements);
+    (void)hipFree(p->count_dev);
+    if (p->num_elements) (void)hipHostFree(p->num_elements);
     p->volume = nullptr;
     p->edep = nullptr;
+    p->count_dev = nullptr;
     p->num_elements = nullptr;
     p->capacity = 0;
+}
+
+int st_pairs_set_count(st_pairs *p, int32_t count)
+{
+    if (!p || !p->count_dev || !p->num_elements || count < 0 || count > p->capacity)
+        return set_error(ST_EINVAL, "st_pairs_set_count: bad arguments");
+    ST_HIP_TRY(hipDeviceSynchronize());
+    ST_HIP_TRY(hipMemcpy(p->count_dev, &count, sizeof count, hipMemcpyHostToDevice));
+    *p->num_elements = count;
+    return ST_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -1000,11 +1267,11 @@ int st_vcs_create(const st_geometry *g, int32_t shift_xy, int32_t shift_z,
     m->suggested_blocks = suggested_blocks;
     m->suggested_threads = suggested_threads;
     m->stream = default_stream();
-    hipError_t e = hipMalloc((void **)&m->ticket, sizeof(unsigned long long));
+    hipError_t e = dev_malloc((void **)&m->ticket, sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMemset(m->ticket, 0, sizeof(unsigned long long));
     if (e != hipSuccess) {
         st_vcs_destroy(m);
-        return set_error(ST_EHIP, "st_vcs_create: %s", hipGetErrorString(e));
+        return set_error(hip_status(e), "st_vcs_create: %s", hipGetErrorString(e));
     }
     *out = m;
     return ST_OK;
@@ -1013,7 +1280,19 @@ int st_vcs_create(const st_geometry *g, int32_t shift_xy, int32_t shift_z,
 int st_vcs_set_stream(st_vcs *m, void *hip_stream)
 {
     if (!m) return set_error(ST_EINVAL, "st_vcs_set_stream: null method");
+    int rc = queue_flush(m);
+    if (rc) return rc;
     m->stream = hip_stream ? (hipStream_t)hip_stream : default_stream();
+    return ST_OK;
+}
+
+int st_vcs_set_queued(st_vcs *m, int32_t on)
+{
+    if (!m) return set_error(ST_EINVAL, "st_vcs_set_queued: null method");
+    int rc = queue_flush(m);
+    if (rc) return rc;
+    m->queue_mode = on != 0;
+    if (!m->queue_mode) m->queue_track = false;
     return ST_OK;
 }
 
@@ -1028,7 +1307,10 @@ int st_vcs_begin_track(st_vcs *m, st_track *track, uint64_t seed, uint32_t track
     if (track->n > 0 && (!track->x || !track->y || !track->z || !track->edep))
         return set_error(ST_EINVAL, "st_vcs_begin_track: track has null arrays");
     const int64_t n = track->n;
-    int rc = grow(&m->lineal, &m->lineal_capacity, std::max<int64_t>(n, 1));
+    int rc = queue_flush(m);   // oversamples left pending by a track that was not ended
+    if (rc) return rc;
+    m->in_track = false;       // until this track is set up
+    rc = grow(&m->lineal, &m->lineal_capacity, std::max<int64_t>(n, 1));
     if (rc) return rc;
     int64_t shifts_cap = m->shifts_capacity;
     rc = grow(&m->shifts, &shifts_cap, std::max<int64_t>(3 * (int64_t)n_oversamples, 3));
@@ -1055,6 +1337,10 @@ int st_vcs_begin_track(st_vcs *m, st_track *track, uint64_t seed, uint32_t track
     m->n_oversamples = n_oversamples;
     m->oversample = 0;
     m->in_track = true;
+    // queued: tracks whose pairs fit the one-workgroup sort and whose pair
+    // rows fit the scratch bound; others run oversample by oversample
+    m->queue_track = m->queue_mode && n >= 1 && n <= kLdsCap &&
+                     n * (int64_t)std::max(n_oversamples, 1) * 16 <= kQueueScratchBytes;
     return ST_OK;
 }
 
@@ -1068,9 +1354,45 @@ int st_vcs_process_oversample(st_vcs *m, const st_track *track, st_pairs *out)
     if (track->n != m->n)
         return set_error(ST_EINVAL, "st_vcs_process_oversample: track has %lld points, begun with %lld",
                          (long long)track->n, (long long)m->n);
-    if (out->capacity < m->n || !out->volume || !out->edep || !out->num_elements)
+    if (out->capacity < m->n || !out->volume || !out->edep || !out->num_elements || !out->count_dev)
         return set_error(ST_EINVAL, "st_vcs_process_oversample: pairs capacity %lld < %lld points",
                          (long long)out->capacity, (long long)m->n);
+    // largest key this method can write: every index is at most N
+    {
+        const unsigned long long N = (unsigned long long)m->g.n_linear;
+        const long double kb = (long double)N + (long double)N * N + (long double)N * N * N;
+        out->key_bound = kb >= 1.8e19L ? 0ull : (N + N * N + N * N * N);
+    }
+    if (m->queue_track) {
+        // record the oversample; it runs when the track ends (or earlier, if
+        // the pending pairs are needed)
+        if (m->queue_active && (m->queue_pairs.volume != out->volume || m->queue_x != track->x ||
+                                m->queue_yc != track->y || m->queue_z != track->z)) {
+            int rc = queue_flush(m);
+            if (rc) return rc;
+        }
+        if (!m->queue_active) {
+            st_vcs *other = queue_producer(out->volume);
+            if (other && other != m) {
+                int rc = queue_flush(other);
+                if (rc) return rc;
+            }
+            m->queue_active = true;
+            m->queue_pairs = *out;
+            m->queue_x = track->x;
+            m->queue_yc = track->y;
+            m->queue_z = track->z;
+            m->queue_os0 = m->oversample;
+            m->queue_count = 0;
+            m->queue_consumers.clear();
+            std::lock_guard<std::mutex> lock(g_queue_mu);
+            g_queue[out->volume] = m;
+        }
+        m->queue_count++;
+        m->queue_consumers.emplace_back();
+        m->oversample++;
+        return ST_OK;
+    }
     const int64_t n = m->n;
     const int tiles = (int)std::max<int64_t>(1, (n + kScoreTile - 1) / kScoreTile);
     ScoreArgs a;
@@ -1079,17 +1401,13 @@ int st_vcs_process_oversample(st_vcs *m, const st_track *track, st_pairs *out)
     a.z = track->z;
     a.lin = m->lineal;
     a.n = n;
-    a.H = m->g.half_length;
-    a.D = m->g.diameter;
-    a.invD = 1.0 / m->g.diameter;
-    a.gso = m->g.gso;
-    a.r2 = m->g.radius * m->g.radius;
-    a.N = (uint64_t)m->g.n_linear;
+    a.g = score_geom(m->g);
     a.shifts = m->shifts;
     a.os = m->oversample;
     a.out_key = out->volume;
     a.out_y = out->edep;
-    a.out_count = out->num_elements;
+    a.out_count = out->count_dev;
+    a.out_count_host = out->num_elements;
     a.status = m->status;
     a.ticket = m->ticket;
     a.ticket_base = m->ticket_base;
@@ -1098,12 +1416,6 @@ int st_vcs_process_oversample(st_vcs *m, const st_track *track, st_pairs *out)
     a.num_tiles = tiles;
     score_kernel<<<tiles, kScoreBlock, 0, m->stream>>>(a);
     ST_HIP_TRY(hipGetLastError());
-    // largest key this method can write: every index is at most N
-    {
-        const unsigned long long N = (unsigned long long)m->g.n_linear;
-        const long double kb = (long double)N + (long double)N * N + (long double)N * N * N;
-        out->key_bound = kb >= 1.8e19L ? 0ull : (N + N * N + N * N * N);
-    }
     m->ticket_base += (uint64_t)tiles;
     m->oversample++;
     return ST_OK;
@@ -1112,15 +1424,20 @@ int st_vcs_process_oversample(st_vcs *m, const st_track *track, st_pairs *out)
 int st_vcs_end_track(st_vcs *m)
 {
     if (!m) return set_error(ST_EINVAL, "st_vcs_end_track: null method");
+    const int rc = queue_flush(m);
     ST_HIP_TRY(hipStreamSynchronize(m->stream));
     m->in_track = false;
-    return ST_OK;
+    return rc;
 }
 
 void st_vcs_destroy(st_vcs *m)
 {
     if (!m) return;
+    (void)queue_flush(m);
     if (m->stream) (void)hipStreamSynchronize(m->stream);
+    (void)hipFree(m->queue_key);
+    (void)hipFree(m->queue_y);
+    if (m->queue_event) (void)hipEventDestroy(m->queue_event);
     (void)hipFree(m->lineal);
     (void)hipFree(m->shifts);
     (void)hipFree(m->status);
@@ -1168,15 +1485,15 @@ int st_hist_create(int32_t type, float lo, float hi, int32_t nbins, st_hist **ou
     h->guess.e0 = h->edges_host[0];
     h->guess.en = h->edges_host[(size_t)nbins];
     h->stream = default_stream();
-    hipError_t e = hipMalloc((void **)&h->edges_dev, ((size_t)nbins + 1) * sizeof(double));
-    if (e == hipSuccess) e = hipMalloc((void **)&h->bins_dev, (size_t)nbins * sizeof(uint64_t));
+    hipError_t e = dev_malloc((void **)&h->edges_dev, ((size_t)nbins + 1) * sizeof(double));
+    if (e == hipSuccess) e = dev_malloc((void **)&h->bins_dev, (size_t)nbins * sizeof(uint64_t));
     if (e == hipSuccess)
         e = hipMemcpy(h->edges_dev, h->edges_host.data(), ((size_t)nbins + 1) * sizeof(double),
                       hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(h->bins_dev, 0, (size_t)nbins * sizeof(uint64_t));
     if (e != hipSuccess) {
         st_hist_destroy(h);
-        return set_error(ST_EHIP, "st_hist_create: %s", hipGetErrorString(e));
+        return set_error(hip_status(e), "st_hist_create: %s", hipGetErrorString(e));
     }
     h->host_counts.resize(nbins);
     *out = h;
@@ -1186,6 +1503,8 @@ int st_hist_create(int32_t type, float lo, float hi, int32_t nbins, st_hist **ou
 int st_hist_set_stream(st_hist *h, void *hip_stream)
 {
     if (!h) return set_error(ST_EINVAL, "st_hist_set_stream: null histogram");
+    int rc = queue_flush_producers(h);
+    if (rc) return rc;
     h->stream = hip_stream ? (hipStream_t)hip_stream : default_stream();
     return ST_OK;
 }
@@ -1212,9 +1531,11 @@ int st_hist_begin_track(st_hist *h, const st_pairs *pairs)
     if (!h || !pairs) return set_error(ST_EINVAL, "st_hist_begin_track: null argument");
     if (pairs->capacity < 0 || pairs->capacity > 0x7FFFFFFF)
         return set_error(ST_EINVAL, "st_hist_begin_track: bad pairs capacity");
+    int rc = queue_flush_producers(h);
+    if (rc) return rc;
     h->capacity = pairs->capacity;
     if (h->capacity > kLdsCap) {
-        int rc = ensure_sort_buffers(h, h->capacity);
+        rc = ensure_sort_buffers(h, h->capacity);
         if (rc) return rc;
     }
     // the per-track counters start at zero (Histogram.cu:139-140)
@@ -1233,6 +1554,17 @@ int st_hist_add(st_hist *h, const st_pairs *pairs)
     if (pairs->capacity > h->capacity)
         return set_error(ST_EINVAL, "st_hist_add: pairs capacity %lld above the begun %lld",
                          (long long)pairs->capacity, (long long)h->capacity);
+    // pairs whose oversample is queued: this histogram consumes it when the
+    // producer runs its queue
+    if (pairs->volume) {
+        st_vcs *m = queue_producer(pairs->volume);
+        if (m && m->queue_active && m->queue_count > 0 && m->queue_pairs.volume == pairs->volume) {
+            m->queue_consumers.back().push_back(h);
+            if (std::find(h->producers.begin(), h->producers.end(), m) == h->producers.end())
+                h->producers.push_back(m);
+            return ST_OK;
+        }
+    }
     return hist_add_pairs(h, pairs, h->stream);
 }
 
@@ -1242,6 +1574,8 @@ int st_hist_end_track(st_hist *h, double *host_bins)
 {
     if (!h) return set_error(ST_EINVAL, "st_hist_end_track: null histogram");
     if (!h->in_track) return set_error(ST_ESTATE, "st_hist_end_track before st_hist_begin_track");
+    int rc = queue_flush_producers(h);
+    if (rc) return rc;
     ST_HIP_TRY(hipMemcpyAsync(h->host_counts.data(), h->bins_dev, (size_t)h->nbins * sizeof(uint64_t),
                               hipMemcpyDeviceToHost, h->stream));
     ST_HIP_TRY(hipStreamSynchronize(h->stream));
@@ -1257,6 +1591,8 @@ int st_hist_end_track(st_hist *h, double *host_bins)
 int st_hist_fetch(st_hist *h, uint64_t *counts, int32_t reset)
 {
     if (!h || !counts) return set_error(ST_EINVAL, "st_hist_fetch: null argument");
+    int rc = queue_flush_producers(h);
+    if (rc) return rc;
     ST_HIP_TRY(hipMemcpyAsync(counts, h->bins_dev, (size_t)h->nbins * sizeof(uint64_t),
                               hipMemcpyDeviceToHost, h->stream));
     ST_HIP_TRY(hipStreamSynchronize(h->stream));
@@ -1272,6 +1608,7 @@ int st_hist_fetch(st_hist *h, uint64_t *counts, int32_t reset)
 void st_hist_destroy(st_hist *h)
 {
     if (!h) return;
+    (void)queue_flush_producers(h);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     (void)hipFree(h->edges_dev);
     (void)hipFree(h->bins_dev);

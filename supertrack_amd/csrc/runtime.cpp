@@ -1,5 +1,6 @@
 // runtime.cpp -- status strings, device memory helpers, geometry and bin
 // edges of libsupertrack_amd (host code).
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
@@ -28,20 +29,56 @@ int set_error(int code, const char *fmt, ...)
 
 void clear_error() { g_last_error.clear(); }
 
+namespace {
+// st_debug_fail_allocation: allocations left before the one that fails (0: off)
+std::atomic<int64_t> g_fail_countdown{0};
+
+bool injected_failure()
+{
+    int64_t c = g_fail_countdown.load();
+    while (c > 0) {
+        if (g_fail_countdown.compare_exchange_weak(c, c - 1)) return c == 1;
+    }
+    return false;
+}
+}  // namespace
+
+hipError_t dev_malloc(void **p, size_t bytes)
+{
+    if (injected_failure()) {
+        *p = nullptr;
+        return hipErrorOutOfMemory;
+    }
+    return hipMalloc(p, bytes);
+}
+
+hipError_t pinned_malloc(void **p, size_t bytes, unsigned int flags)
+{
+    if (injected_failure()) {
+        *p = nullptr;
+        return hipErrorOutOfMemory;
+    }
+    return hipHostMalloc(p, bytes, flags);
+}
+
+namespace {
+// per-device default streams, created on first use, destroyed by st_shutdown
+std::mutex g_stream_mu;
+std::vector<hipStream_t> g_streams;
+}  // namespace
+
 hipStream_t default_stream()
 {
-    static std::mutex mu;
-    static std::vector<hipStream_t> streams;
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return nullptr;
-    std::lock_guard<std::mutex> lock(mu);
-    if ((int)streams.size() <= dev) streams.resize(dev + 1, nullptr);
-    if (!streams[dev]) {
+    std::lock_guard<std::mutex> lock(g_stream_mu);
+    if ((int)g_streams.size() <= dev) g_streams.resize(dev + 1, nullptr);
+    if (!g_streams[dev]) {
         hipStream_t s = nullptr;
         if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return nullptr;
-        streams[dev] = s;
+        g_streams[dev] = s;
     }
-    return streams[dev];
+    return g_streams[dev];
 }
 
 int device_cu_count()
@@ -119,7 +156,7 @@ int st_malloc(void **ptr, size_t bytes)
     if (!ptr) return set_error(ST_EINVAL, "st_malloc: null pointer");
     *ptr = nullptr;
     if (bytes == 0) return ST_OK;
-    hipError_t e = hipMalloc(ptr, bytes);
+    hipError_t e = dev_malloc(ptr, bytes);
     if (e != hipSuccess) {
         *ptr = nullptr;
         return set_error(ST_ENOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
@@ -175,6 +212,32 @@ int st_device_synchronize(void)
 {
     ST_HIP_TRY(hipDeviceSynchronize());
     return ST_OK;
+}
+
+int st_debug_fail_allocation(int64_t countdown)
+{
+    if (countdown < 0) return set_error(ST_EINVAL, "st_debug_fail_allocation: countdown < 0");
+    g_fail_countdown.store(countdown);
+    return ST_OK;
+}
+
+int st_shutdown(void)
+{
+    std::lock_guard<std::mutex> lock(g_stream_mu);
+    int cur = 0;
+    const bool have_cur = hipGetDevice(&cur) == hipSuccess;
+    int rc = ST_OK;
+    for (int dev = 0; dev < (int)g_streams.size(); ++dev) {
+        if (!g_streams[dev]) continue;
+        hipError_t e = hipSetDevice(dev);
+        if (e == hipSuccess) e = hipStreamSynchronize(g_streams[dev]);
+        if (e == hipSuccess) e = hipStreamDestroy(g_streams[dev]);
+        if (e != hipSuccess && rc == ST_OK)
+            rc = set_error(ST_EHIP, "st_shutdown: device %d: %s", dev, hipGetErrorString(e));
+        g_streams[dev] = nullptr;
+    }
+    if (have_cur) (void)hipSetDevice(cur);
+    return rc;
 }
 
 int st_default_stream(void **hip_stream)

@@ -30,6 +30,27 @@ void clear_error();
 // first use and kept for the life of the process.
 hipStream_t default_stream();
 
+// Every device and pinned-host allocation of the library goes through these,
+// so that st_debug_fail_allocation can make the n-th one fail (with
+// hipErrorOutOfMemory, as a full device would).
+hipError_t dev_malloc(void **p, size_t bytes);
+hipError_t pinned_malloc(void **p, size_t bytes, unsigned int flags);
+
+// Status for a failed HIP call: ST_ENOMEM for an allocation failure.
+inline int hip_status(hipError_t e)
+{
+    return (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? ST_ENOMEM : ST_EHIP;
+}
+
+// Returns ST_ENOMEM (with the failing call in the message) if expr fails.
+#define ST_ALLOC_TRY(expr)                                                              \
+    do {                                                                                \
+        hipError_t st_alloc_err_ = (expr);                                              \
+        if (st_alloc_err_ != hipSuccess)                                                \
+            return ::st::set_error(ST_ENOMEM, "%s failed: %s (%s:%d)", #expr,            \
+                                   hipGetErrorString(st_alloc_err_), __FILE__, __LINE__); \
+    } while (0)
+
 // ---- Philox4x32-10 (Salmon et al., SC'11) ------------------------------------
 
 struct u32x4 {

@@ -6,12 +6,11 @@
 #include <vector>
 
 #include "st_common.hpp"
+#include "parallel.hpp"
 
 namespace st {
 
 int device_cu_count();
-int host_threads();
-void parallel_for(int64_t n, const std::function<void(int64_t)> &fn);
 void make_geometry(double half_length_ini, double diameter_ini, st_geometry *g);
 void make_edges(int type, float lo, float hi, int nbins, double *edges);
 
@@ -71,6 +70,8 @@ struct st_hist {
     unsigned long long os_ticket_base = 0;
     uint32_t os_epoch = 0;
     std::vector<uint64_t> host_counts;
+    // methods with queued oversamples whose pairs this histogram consumed
+    std::vector<st_vcs *> producers;
 };
 
 // VoxelConstrainedSphereMethod object (VoxelConstrainedSphereMethod.hh:43-75).
@@ -94,4 +95,22 @@ struct st_vcs {
     unsigned long long *ticket = nullptr;
     uint64_t ticket_base = 0;
     uint32_t epoch = 0;
+    // queued oversamples (st_vcs_set_queued): the oversamples of a track are
+    // recorded and run as one launch when the track ends
+    bool queue_mode = false;     // asked for by the caller
+    bool queue_track = false;    // the current track is queued
+    bool queue_active = false;   // oversamples are pending
+    st_pairs queue_pairs{};      // the caller's pairs of the pending oversamples
+    int32_t queue_os0 = 0, queue_count = 0;
+    std::vector<std::vector<st_hist *>> queue_consumers;   // per pending oversample
+    const double *queue_x = nullptr, *queue_yc = nullptr, *queue_z = nullptr;   // the track's device arrays
+    uint64_t *queue_key = nullptr;   // scratch rows of n pairs, one per pending oversample
+    double *queue_y = nullptr;
+    int64_t queue_key_capacity = 0, queue_y_capacity = 0;
+    hipEvent_t queue_event = nullptr;
 };
+
+namespace st {
+// Runs the pending queued oversamples of m (no-op if none).
+int queue_flush(st_vcs *m);
+}  // namespace st

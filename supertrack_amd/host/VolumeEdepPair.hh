@@ -14,7 +14,7 @@ public:
 
     uint64_t *volume = nullptr;
     double *edep = nullptr;
-    int *numElements = nullptr;   // device counter
+    int *numElements = nullptr;   // pair count: pinned host memory, read after the track's sync
 
     st_pairs c{};
 };

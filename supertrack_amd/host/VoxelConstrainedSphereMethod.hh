@@ -3,7 +3,9 @@
 // Keys: [VoxelConstrainedSphere] ScoringRegionHalfLength, ScoringSphereDiameter,
 // ShiftTrack (default true), SuggestedCudaBlocks, SuggestedCudaThreads, and
 // the additive ShiftTrackZ (default false: z is never shifted, as in the
-// reference).
+// reference) and QueueOversamples (default false: each ProcessTrack call
+// launches its work; true: a track's oversamples run as one launch when the
+// track ends, st_vcs_set_queued).
 #pragma once
 
 #include "SimulationMethod.hh"
@@ -33,6 +35,7 @@ private:
     int _suggestedCudaBlocks = 0;
     bool _randomShifts = true;
     bool _randomShiftsZ = false;
+    bool _queueOversamples = false;
     bool _referenceSeeds = true;
     SphericalGeometry _sphericalGeometry;
     st_vcs *_vcs = nullptr;

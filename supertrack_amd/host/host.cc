@@ -311,6 +311,7 @@ VoxelConstrainedSphereMethod::VoxelConstrainedSphereMethod(const INIReader &read
     const st_geometry g = _sphericalGeometry.c();
     check(st_vcs_create(&g, _randomShifts, _randomShiftsZ, _suggestedCudaBlocks, _suggestedCudaThreads, &_vcs),
           "VoxelConstrainedSphereMethod");
+    if (_queueOversamples) check(st_vcs_set_queued(_vcs, 1), "VoxelConstrainedSphereMethod");
 }
 
 VoxelConstrainedSphereMethod::~VoxelConstrainedSphereMethod() { st_vcs_destroy(_vcs); }
@@ -321,6 +322,7 @@ void VoxelConstrainedSphereMethod::ParseInput()
     _suggestedCudaThreads = (int)_macroReader.GetReal("VoxelConstrainedSphere", "SuggestedCudaThreads", 0);
     _randomShifts = _macroReader.GetBoolean("VoxelConstrainedSphere", "ShiftTrack", true);
     _randomShiftsZ = _macroReader.GetBoolean("VoxelConstrainedSphere", "ShiftTrackZ", false);
+    _queueOversamples = _macroReader.GetBoolean("VoxelConstrainedSphere", "QueueOversamples", false);
     _referenceSeeds = _macroReader.GetBoolean("Run", "ReferenceSeeds", true);
 }
 

@@ -64,5 +64,11 @@ int main(int argc, char **argv)
     std::cout << "Scored spheres: " << out.Integral() << std::endl;
     std::cout << "Run time: " << std::chrono::duration<double, std::milli>(t1 - t0).count() << " ms" << std::endl;
     std::cout << "SuperTrack concluding after: " << (time(nullptr) - start) << " seconds." << std::endl;
+    // every device object of the run is gone: release the library's streams
+    // before the HIP runtime is torn down at exit
+    if (st_shutdown() != ST_OK) {
+        std::cerr << "st_shutdown: " << st_last_error() << std::endl;
+        return 1;
+    }
     return 0;
 }

@@ -40,6 +40,10 @@ def gpu_run(x, y, z, e, off, geoms_hd, gt, n_os, seed, shift_xy, shift_z, precis
     counts = [h.fetch() for h in hs]
     info = lib.info()
     info.update(b.stats())
+    bal = b.balance()
+    # conservation: every energy word and entry that entered a sphere table left it
+    assert bal["energy_in"] == bal["energy_out"] and bal["entries_created"] == bal["entries_retired"], bal
+    assert (bal["entries_created"] > 0) == (info["points_in_sphere"] > 0)
     edges = [h.edges.copy() for h in hs]
     b.free()
     for h in hs:
@@ -476,16 +480,21 @@ def test_batch_full_size_library_invariants():
                         blocks_per_cu=bpc, points_per_thread=1)
         b.run()
         b.sync()
-        results.append(([h.fetch() for h in hs], b.stats()))
+        bal = b.balance()
+        assert bal["energy_in"] == bal["energy_out"] and bal["entries_created"] == bal["entries_retired"], bal
+        results.append(([h.fetch() for h in hs], b.stats(), bal))
         b.free()
         for h in hs:
             h.free()
     lib.free()
-    (c0, s0), (c1, s1), (c2, s2) = results
+    (c0, s0, k0), (c1, s1, k1), (c2, s2, k2) = results
     for a, b2, c in zip(c0, c1, c2):
         np.testing.assert_array_equal(a, b2)
         np.testing.assert_array_equal(a, c)
     assert s0 == s1 == s2
+    assert k0 == k1 == k2
+    # every binned count is a retired entry (entries outside the histogram are retired unbinned)
+    assert sum(int(a.sum()) for a in c0) <= k0["entries_retired"] <= s0["points_in_sphere"]
     assert 0 < s0["points_in_sphere"] <= s0["points_scored"] <= deposits
     assert s0["fp64_rechecks"] < s0["points_scored"] // 1000
     total = sum(int(a.sum()) for a in c0)
@@ -518,3 +527,33 @@ def test_batch_fp64_whole_cfg3_against_oracle():
           flush=True)
     assert total > 10**8
     assert diff <= 2
+
+
+@pytest.mark.parametrize("name,stride", [("cfg4", 10), ("cfg5", 96)])
+def test_batch_bench_sample_against_oracle(name, stride):
+    """The sample that the benchmark's CPU baseline times -- every stride-th
+    track of the full library with all 1000 oversamples, every geometry and
+    histogram -- scored by the batched FP32 kernel and by the oracle's run
+    driver.  Every count is equal except where a sphere's oracle sum lies
+    within 1e-6 of an edge (the FP32 energies' error bound): each such sum
+    may move one count to the neighbouring bin, changing two counts."""
+    w = W.workload(name)
+    n_sample = (w["n_tracks"] + stride - 1) // stride
+    x, y, z, e, off, gt = W.make_library(w, n_tracks=n_sample, stride=stride)
+    mixed = len(w["geometries"]) > 1
+    print(f"\nbench sample {name}: {n_sample} tracks, {int(off[-1])} points x {w['n_oversamples']} oversamples",
+          flush=True)
+    counts, _, info, edges = gpu_run(x, y, z, e, off, w["geometries"], gt if mixed else None, w["n_oversamples"],
+                                     W.SEED, True, True, w["precision"], w["hists"], use_graph=True)
+    assert info["exact_tracks"] == 0
+    want, secs, threads, near = po.run(x, y, z, e, off, [po.geometry(*g) for g in w["geometries"]],
+                                       gt if mixed else None, w["n_oversamples"], W.SEED, True, True, edges,
+                                       near_rtol=1e-6)
+    total = 0
+    for h, (g, v) in enumerate(zip(counts, want)):
+        diff = int(np.abs(g.astype(np.int64) - v.astype(np.int64)).sum())
+        total += int(v.sum())
+        print(f"PARITY bench sample {name} hist {h}: {int(v.sum())} counts, {diff} differ, {near[h]} sums within "
+              f"1e-6 of an edge (oracle {secs:.1f} s on {threads} threads)", flush=True)
+        assert diff <= 2 * near[h], (h, diff, near[h])
+    assert total > 10**7

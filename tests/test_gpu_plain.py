@@ -325,3 +325,179 @@ def test_pairs_degenerate_and_fine_lattices(H, D):
     e = rng.uniform(1, 1000, n)
     total = assert_pairs_equal(x, y, z, e, H, D, 8, 3, 3, True, True)
     assert total > 100
+
+
+def test_num_elements_is_host_readable_after_the_track_sync():
+    """VolumeEdepPair::numElements is read on the host after the track's
+    synchronisation (VolumeEdepPair.cu:10-13, Histogram.cu:105): through the C
+    ABI, after st_vcs_end_track the count is read through the plain pointer,
+    and equals the oracle's pair count of the last oversample."""
+    import ctypes as C
+    from supertrack_amd import _capi
+    w = W.workload("cfg1")
+    x, y, z, e, off, _ = W.make_library(w)
+    n = len(x)
+    g = _capi.Geometry()
+    st.call("st_geometry_make", C.c_double(2.5e3), C.c_double(10.0), C.byref(g))
+    m = C.c_void_p()
+    st.call("st_vcs_create", C.byref(g), 1, 1, 0, 0, C.byref(m))
+    trk = _capi.Track()
+    arrs = [np.ascontiguousarray(a, np.float64) for a in (x, y, z, e)]
+    st.call("st_track_upload", C.byref(trk), *[a.ctypes.data_as(_capi.c_double_p) for a in arrs], C.c_int64(n))
+    pairs = _capi.Pairs()
+    st.call("st_pairs_alloc", C.byref(pairs), C.c_int64(n))
+    count = C.cast(pairs.num_elements, C.POINTER(C.c_int32))
+    assert count[0] == n                       # starts at the capacity
+    n_os = 6
+    want = po.track_segments(x, y, z, e, po.geometry(2.5e3, 10.0), 5, 0, n_os, True, True)
+    assert sum(len(wk) for wk, _, _, _ in want) > 0
+    st.call("st_vcs_begin_track", m, C.byref(trk), C.c_uint64(5), C.c_uint32(0), C.c_int32(n_os))
+    for k in range(n_os):
+        st.call("st_vcs_process_oversample", m, C.byref(trk), C.byref(pairs))
+        if k < n_os - 1:
+            st.call("st_device_synchronize")
+            assert count[0] == len(want[k][0]), k
+    st.call("st_vcs_end_track", m)
+    assert count[0] == len(want[-1][0])
+    st.lib().st_pairs_free(C.byref(pairs))
+    st.lib().st_track_free(C.byref(trk))
+    st.lib().st_vcs_destroy(m)
+
+
+def test_process_exit_with_live_objects_is_clean(tmp_path):
+    """A program that leaves every kind of library object alive (and work
+    queued) at exit ends with status 0: the binding frees them and calls
+    st_shutdown before the HIP runtime is torn down."""
+    import subprocess
+    import sys
+    import os
+    script = tmp_path / "live_objects.py"
+    script.write_text(f"""
+import sys
+sys.path.insert(0, {os.path.dirname(os.path.dirname(os.path.abspath(__file__)))!r})
+import numpy as np
+import supertrack_amd as st
+from supertrack_amd import workloads as W
+w = W.workload("cfg2")
+x, y, z, e, off, _ = W.make_library(w, n_tracks=4)
+g = st.SphericalGeometry(2.5e3, 10.0)
+h = st.Histogram(3000, 0.01, 300.0, "lin")
+st.run_plain(x, y, z, e, off, g, h, 3, 1, True, True)
+m = st.VoxelConstrainedSphereMethod(g, True, True)
+trk = st.Track(x[:off[1]], y[:off[1]], z[:off[1]], e[:off[1]])
+pairs = st.VolumeEdepPair(int(off[1]))
+m.allocate_track_process(trk, 1, 0, 5)
+h.allocate_track_process(pairs)
+m.process_track(trk, pairs)
+h.sort_reduce_and_add_to_histogram(pairs)
+lib = st.TrackLibrary(x, y, z, e, off, [g], None, st.ST_F32)
+hb = st.Histogram(1000, 1e-2, 1e3, "log")
+b = st.BatchRun(lib, [hb], 50, 1, True, True)
+b.run()
+d = st.DeviceArray(1 << 20)
+print("objects alive at exit", flush=True)
+""")
+    res = subprocess.run([sys.executable, str(script)], capture_output=True, text=True, timeout=300)
+    assert res.returncode == 0, (res.returncode, res.stdout[-2000:], res.stderr[-2000:])
+    assert "objects alive at exit" in res.stdout
+
+
+@pytest.mark.parametrize("shift_z", [False, True])
+def test_queued_oversamples_equal_the_per_oversample_path(shift_z):
+    """Queued oversamples (one launch per track at its end) give the same bins
+    as the per-oversample path and the oracle, bit for bit, on cfg2 tracks
+    (including one above the 12288-point queue limit, which runs oversample
+    by oversample); after the track the pairs hold the last oversample's."""
+    w = W.workload("cfg2")
+    x, y, z, e, off, _ = W.make_library(w, n_tracks=6)
+    big = st.synth_library(dict(W.workload("cfg3")["synth"], n_min=20000, n_max=20000), 1, 3)
+    x = np.concatenate([x, big[0]]); y = np.concatenate([y, big[1]])
+    z = np.concatenate([z, big[2]]); e = np.concatenate([e, big[3]])
+    off = np.concatenate([off, [off[-1] + len(big[0])]]).astype(np.int64)
+    assert int(off[-1] - off[-2]) > 12288
+    g = st.SphericalGeometry(2.5e3, 10.0)
+    n_os = 9
+    hq = st.Histogram(3000, 0.01, 300.0, "lin")
+    hp = st.Histogram(3000, 0.01, 300.0, "lin")
+    st.run_plain(x, y, z, e, off, g, hq, n_os, 7, True, shift_z, queued=True)
+    st.run_plain(x, y, z, e, off, g, hp, n_os, 7, True, shift_z)
+    ed = po.edges("lin", 0.01, 300.0, 3000)
+    bins, _, _ = po.run(x, y, z, e, off, [po.geometry(2.5e3, 10.0)], None, n_os, 7, True, shift_z, [ed])
+    np.testing.assert_array_equal(hq.counts(), bins[0].astype(np.float64))
+    np.testing.assert_array_equal(hp.counts(), bins[0].astype(np.float64))
+    assert bins[0].sum() > 1000
+    # the pairs after a queued track: the last oversample's, in point order
+    t = 2
+    b, en = off[t], off[t + 1]
+    m = st.VoxelConstrainedSphereMethod(g, True, shift_z)
+    m.set_queued(True)
+    trk = st.Track(x[b:en], y[b:en], z[b:en], e[b:en])
+    pairs = st.VolumeEdepPair(int(en - b))
+    h = st.Histogram(3000, 0.01, 300.0, "lin")
+    m.allocate_track_process(trk, 7, t, n_os)
+    h.allocate_track_process(pairs)
+    for _ in range(n_os):
+        m.process_track(trk, pairs)
+        h.sort_reduce_and_add_to_histogram(pairs)
+    m.free_track_process()
+    h.free_track_process()
+    want = po.track_segments(x[b:en], y[b:en], z[b:en], e[b:en], po.geometry(2.5e3, 10.0), 7, t, n_os, True,
+                             shift_z)
+    gk, gy = pairs.download()
+    np.testing.assert_array_equal(gk, want[-1][0])
+    np.testing.assert_array_equal(gy, want[-1][1])
+    for o in (m, h, trk, pairs):
+        o.free()
+
+
+def test_queued_oversamples_uneven_consumers_and_streams():
+    """Two histograms consuming different oversamples of one queued track, one
+    of them on another stream; a histogram read before the track ends sees
+    the pending work; pairs freed while oversamples are pending."""
+    import ctypes as C
+    w = W.workload("cfg2")
+    x, y, z, e, off, _ = W.make_library(w, n_tracks=1)
+    n = int(off[1])
+    g = st.SphericalGeometry(2.5e3, 10.0)
+    gp = po.geometry(2.5e3, 10.0)
+    n_os = 7
+    segs = po.track_segments(x, y, z, e, gp, 3, 0, n_os, True, True)
+    ed_lin = po.edges("lin", 0.01, 300.0, 3000)
+    ed_log = po.edges("log", 1e-2, 1e3, 1000)
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipStreamCreate.argtypes = [C.POINTER(C.c_void_p)]
+    hip.hipStreamDestroy.argtypes = [C.c_void_p]
+    other = C.c_void_p()
+    assert hip.hipStreamCreate(C.byref(other)) == 0
+    m = st.VoxelConstrainedSphereMethod(g, True, True)
+    m.set_queued(True)
+    trk = st.Track(x, y, z, e)
+    pairs = st.VolumeEdepPair(n)
+    ha = st.Histogram(3000, 0.01, 300.0, "lin")
+    hb = st.Histogram(1000, 1e-2, 1e3, "log")
+    hb.set_stream(other)
+    m.allocate_track_process(trk, 3, 0, n_os)
+    ha.allocate_track_process(pairs)
+    hb.allocate_track_process(pairs)
+    for k in range(n_os):
+        m.process_track(trk, pairs)
+        ha.sort_reduce_and_add_to_histogram(pairs)          # every oversample
+        if k % 3 == 1:
+            hb.sort_reduce_and_add_to_histogram(pairs)      # oversamples 1 and 4 only
+            hb.sort_reduce_and_add_to_histogram(pairs)      # twice: counted twice, as on the per-oversample path
+    # reading hb before the method's track ends runs the pending oversamples
+    got_b = hb.fetch(reset=False)
+    want_b = sum(2 * po.histogram_counts(ed_log, segs[k][3]) for k in (1, 4))
+    np.testing.assert_array_equal(got_b, want_b)
+    m.free_track_process()
+    ha.free_track_process()
+    want_a = sum(po.histogram_counts(ed_lin, s[3]) for s in segs)
+    np.testing.assert_array_equal(ha.counts(), want_a.astype(np.float64))
+    # pending oversamples, then the pairs are freed: they run first
+    m.allocate_track_process(trk, 3, 1, 2)
+    m.process_track(trk, pairs)
+    pairs.free()
+    m.free_track_process()
+    for o in (m, trk, ha, hb):
+        o.free()
+    hip.hipStreamDestroy(other)

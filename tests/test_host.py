@@ -9,7 +9,7 @@ import pytest
 from supertrack_amd import trackfile
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-CLI = os.path.join(ROOT, "supertrack_amd", "supertrack")
+CLI = os.environ.get("SUPERTRACK_AMD_CLI") or os.path.join(ROOT, "supertrack_amd", "supertrack")
 
 
 def ini_dump(tmp_path, text, section, name):

@@ -283,3 +283,42 @@ def test_energy_conservation():
         pk, py, keys, sums = po.score_oversample(x, y, z + g.half_length, e, g, *sh)
         assert math.fsum(sums) == pytest.approx(math.fsum(py), rel=1e-12)
         assert len(keys) == len(np.unique(pk))
+
+
+def test_near_edge_matches_brute_force():
+    # orc_near_edge against a direct scan of every edge (an edge within
+    # rtol * |v| of v is adjacent to v for these rtol, since edges are
+    # further apart than that), including values on and beside edges
+    rng = np.random.default_rng(42)
+    for kind, nb, lo, hi in [("lin", 3000, 0.01, 300.0), ("log", 1000, 1e-2, 1e3)]:
+        e = po.edges(kind, lo, hi, nb)
+        vals = np.concatenate([
+            rng.uniform(0.005, 1.2 * hi, 4000),
+            e[rng.integers(0, nb + 1, 500)],
+            e[rng.integers(0, nb + 1, 500)] * (1 + rng.choice([-1, 1], 500) * rng.uniform(0, 3e-6, 500)),
+            [np.nan, np.inf, -1.0, 0.0]])
+        for rtol in (1e-6, 1e-12):
+            for v in vals:
+                want = bool(not np.isnan(v) and np.any(np.abs(v - e) <= rtol * abs(v)))
+                assert po.near_edge(e, float(v), rtol) == want, (kind, v, rtol)
+
+
+def test_run_counts_near_edge_sums():
+    # the run driver's near-edge totals equal orc_near_edge applied to every
+    # sum of every segment, and its bins equal the per-segment sums binned
+    from supertrack_amd import workloads as W
+    import supertrack_amd as st
+    w = W.workload("cfg3")
+    spec = dict(w["synth"], n_max=20000)
+    x, y, z, e, off = st.synth_library(spec, 6, 0)
+    g = po.geometry(2.5e3, 10.0)
+    edges = [po.edges("lin", 0.01, 300.0, 3000), po.edges("log", 1e-2, 1e3, 1000)]
+    bins, _, _, near = po.run(x, y, z, e, off, [g], None, 5, 7, True, True, edges, near_rtol=1e-3)
+    sums = np.concatenate([s for t in range(6)
+                           for _, _, _, s in po.track_segments(x[off[t]:off[t + 1]], y[off[t]:off[t + 1]],
+                                                               z[off[t]:off[t + 1]], e[off[t]:off[t + 1]], g, 7, t,
+                                                               5, True, True)])
+    for h in range(2):
+        np.testing.assert_array_equal(bins[h], po.histogram_counts(edges[h], sums))
+        assert near[h] == sum(po.near_edge(edges[h], float(v), 1e-3) for v in sums)
+    assert near[0] > 0 and near[1] > 0

@@ -9,7 +9,7 @@ import pytest
 import pyoracle as po
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-CLI = os.path.join(ROOT, "supertrack_amd", "supertrack")
+CLI = os.environ.get("SUPERTRACK_AMD_CLI") or os.path.join(ROOT, "supertrack_amd", "supertrack")
 
 pytestmark = pytest.mark.skipif(not os.path.exists(CLI), reason="supertrack CLI not built")
 
