@@ -278,6 +278,25 @@ int st_batch_record_fetch(st_batch *b, int64_t *segments, uint64_t *keys, double
                           int64_t *count);
 void st_batch_destroy(st_batch *b);
 
+/* Allocation-free pass.  No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  The same pass as st_batch_create +
+ * st_batch_run over a finished library, adding counts to each histogram's
+ * device counters, but it allocates nothing: its control block (segment
+ * ticket, error flag, run statistics) is a 256-byte device array of the
+ * library, one per device, held until the process ends.  Launch shape as
+ * st_batch_create without autotune (opts->autotune must be 0; use_graph is
+ * ignored).  Limit: the library may hold no track routed to the exact
+ * per-segment path (st_library_info_get: exact_tracks == 0), else ST_ERANGE.
+ * hip_stream NULL = the library's default stream.  stats7 NULL: the call only
+ * queues the pass and returns.  Otherwise it waits for the pass, writes
+ * stats7[0..2] as st_batch_stats and [3..6] as st_batch_balance, and returns
+ * ST_ESTATE if the kernel reported an error or the conservation totals differ.
+ * Calls on one device share that device's control block: calls that overlap
+ * in time on one device (other streams or host threads) race and must not be
+ * issued. */
+int st_batch_run_static(st_library *lib, const st_batch_opts *opts, st_hist *const *hists, int32_t n_hists,
+                        void *hip_stream, int64_t *stats7);
 
 /* ---- synthetic track library (deterministic, Philox-driven) ---- */
 typedef struct {

@@ -122,6 +122,8 @@ PROTOTYPES = {
     "st_batch_times": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
     "st_batch_stats": (C.c_int, [C.c_void_p, c_int64_p]),
     "st_batch_balance": (C.c_int, [C.c_void_p, c_uint64_p]),
+    "st_batch_run_static": (C.c_int, [C.c_void_p, C.POINTER(BatchOpts), C.POINTER(C.c_void_p), C.c_int32,
+                                      C.c_void_p, c_int64_p]),
     "st_batch_record": (C.c_int, [C.c_void_p, C.c_int64]),
     "st_batch_record_fetch": (C.c_int, [C.c_void_p, c_int64_p, c_uint64_p, c_double_p, c_int64_p]),
     "st_batch_destroy": (None, [C.c_void_p]),

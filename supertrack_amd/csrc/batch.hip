@@ -33,6 +33,7 @@
 #include <cmath>
 #include <cstring>
 #include <deque>
+#include <mutex>
 #include <new>
 #include <numeric>
 #include <type_traits>
@@ -1054,6 +1055,40 @@ int autotune(st_batch *b)
     return configure(b, best_s, best_b, best_p);
 }
 
+// ---------------------------------------------------------------------------
+// allocation-free pass (st_batch_run_static)
+// ---------------------------------------------------------------------------
+// Its control block lives in this file-scope device array: one copy per
+// device, kept until the process ends.  The host-side name is a stand-in, not
+// a device address: each device's address comes from hipGetSymbolAddress,
+// taken once per device with that device current, and only that address is
+// handed to kernels and copies.
+constexpr size_t kStaticScratchBytes = 256;
+static_assert(sizeof(BatchCtl) <= kStaticScratchBytes, "the control block fits the static scratch");
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+constexpr int kMaxDevices = 64;
+
+int static_scratch(BatchCtl **out)
+{
+    static std::mutex mu;
+    static void *addr[kMaxDevices] = {};
+    int dev = 0;
+    ST_HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices)
+        return set_error(ST_ERANGE, "st_batch_run_static: device %d beyond the %d supported", dev, kMaxDevices);
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return set_error(ST_EHIP, "st_batch_run_static: hipGetSymbolAddress(scratch) failed on device %d: %s",
+                             dev, hipGetErrorString(e));
+        addr[dev] = p;
+    }
+    *out = reinterpret_cast<BatchCtl *>(addr[dev]);
+    return ST_OK;
+}
+
 // Waits for the work queued on `stream` and copies the control block back.
 int read_ctl(const BatchCtl *ctl, hipStream_t stream, BatchCtl *out)
 {
@@ -1323,6 +1358,47 @@ int st_batch_create(st_library *lib, const st_batch_opts *opts, st_hist *const *
     }
     *out = b;
     return ST_OK;
+}
+
+int st_batch_run_static(st_library *lib, const st_batch_opts *opts, st_hist *const *hists, int32_t n_hists,
+                        void *hip_stream, int64_t *stats7)
+{
+    if (!lib || !opts || !hists || n_hists < 1 || n_hists > kMaxHists)
+        return set_error(ST_EINVAL, "st_batch_run_static: need a library, options and 1..%d histograms", kMaxHists);
+    if (!lib->finished) return set_error(ST_EINVAL, "st_batch_run_static: library not finished (st_library_finish)");
+    if (opts->n_oversamples < 0) return set_error(ST_EINVAL, "st_batch_run_static: n_oversamples < 0");
+    if (opts->autotune)
+        return set_error(ST_EINVAL, "st_batch_run_static: autotune times candidate passes on buffers it "
+                                    "allocates; tune with st_batch_create and pass the chosen shape");
+    if (!lib->exact.empty())
+        return set_error(ST_ERANGE,
+                         "st_batch_run_static: %zu tracks of the library need the exact per-segment path, "
+                         "whose per-track buffers are allocated (limit: 0 such tracks); use st_batch_run",
+                         lib->exact.size());
+    int hist_words = 0;
+    for (int i = 0; i < n_hists; ++i) {
+        if (!hists[i]) return set_error(ST_EINVAL, "st_batch_run_static: null histogram %d", i);
+        hist_words += hists[i]->nbins;
+    }
+    if (stats7)
+        for (int i = 0; i < kStats; ++i) stats7[i] = 0;
+    if (opts->n_oversamples == 0) return ST_OK;
+    Launch ln;
+    int rc = default_launch(lib->precision, hist_words, *opts, &ln);
+    if (rc) return rc;
+    BatchCtl *ctl = nullptr;
+    rc = static_scratch(&ctl);
+    if (rc) return rc;
+    hipStream_t stream = hip_stream ? (hipStream_t)hip_stream : default_stream();
+    if (!stream) return set_error(ST_EHIP, "st_batch_run_static: no stream");
+    const BatchArgs a = make_args(lib, *opts, hists, n_hists, ln, ctl);
+    rc = launch_pass(lib->precision, a, ln, stream);
+    if (rc || !stats7) return rc;
+    BatchCtl c;
+    rc = read_ctl(ctl, stream, &c);
+    if (rc) return rc;
+    for (int i = 0; i < kStats; ++i) stats7[i] = (int64_t)c.stats[i];
+    return check_ctl(c);
 }
 
 int st_batch_set_stream(st_batch *b, void *hip_stream)

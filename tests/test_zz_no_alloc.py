@@ -1,0 +1,112 @@
+"""st_batch_run_static, the allocation-free pass, against st_batch_run.
+
+Both run the same fused kernel over the same library with the same options,
+so every histogram count and run statistic must be identical.  Two calls in a
+row, the second on a smaller library (fewer tracks, fewer oversamples), show
+that the shared device control block is cleared between calls.  Called
+through the C ABI only."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import supertrack_amd as st
+from supertrack_amd import _capi
+from supertrack_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+HISTS = [("lin", 3000, 0.01, 300.0), ("log", 1000, 1e-2, 1e3)]
+
+
+def _opts(n_os, seed=77):
+    o = _capi.BatchOpts()
+    o.n_oversamples = n_os
+    o.shift_xy = 1
+    o.shift_z = 1
+    o.track_index_base = 0
+    o.seed = seed
+    o.use_graph = 0
+    o.autotune = 0
+    return o
+
+
+def _library(n_tracks, precision, t0=0):
+    w = W.workload("cfg3")
+    spec = dict(w["synth"], n_max=60000)
+    x, y, z, e, off = st.synth_library(spec, n_tracks, t0)
+    return st.TrackLibrary(x, y, z, e, off, [st.SphericalGeometry(2.5e3, 10.0)], None, precision)
+
+
+def _reference(lib, n_os):
+    hs = [st.Histogram(nb, lo, hi, kind) for kind, nb, lo, hi in HISTS]
+    b = st.BatchRun(lib, hs, n_os, 77, True, True, use_graph=False)
+    b.run()
+    b.sync()
+    stats = b.stats()
+    bal = b.balance()
+    counts = [h.fetch() for h in hs]
+    b.free()
+    for h in hs:
+        h.free()
+    return counts, stats, bal
+
+
+def _static(lib, n_os):
+    hs = [st.Histogram(nb, lo, hi, kind) for kind, nb, lo, hi in HISTS]
+    harr = (C.c_void_p * len(hs))(*[h.h for h in hs])
+    stats = np.zeros(7, np.int64)
+    o = _opts(n_os)
+    print(f"st_batch_run_static: {lib.info()['n_tracks']} tracks x {n_os} oversamples", flush=True)
+    st.call("st_batch_run_static", lib.h, C.byref(o), harr, len(hs), None, stats.ctypes.data_as(_capi.c_int64_p))
+    counts = [h.fetch() for h in hs]
+    for h in hs:
+        h.free()
+    return counts, stats
+
+
+@pytest.mark.parametrize("precision", [st.ST_F32, st.ST_F64])
+def test_static_pass_equals_batch_run_twice(precision):
+    big = _library(24, precision)
+    small = _library(7, precision, t0=100)
+    try:
+        for lib, n_os in ((big, 6), (small, 3)):
+            want, wstats, wbal = _reference(lib, n_os)
+            got, stats = _static(lib, n_os)
+            for g, w in zip(got, want):
+                np.testing.assert_array_equal(g, w)
+            assert sum(int(g.sum()) for g in got) > 100
+            assert stats[0] == wstats["points_scored"]
+            assert stats[1] == wstats["fp64_rechecks"]
+            assert stats[2] == wstats["points_in_sphere"]
+            # conservation totals, and equal to the object-based run's
+            u = stats.view(np.uint64)
+            assert u[3] == u[4] and u[5] == u[6] and u[5] > 0
+            assert (int(u[3]), int(u[5])) == (wbal["energy_in"], wbal["entries_created"])
+    finally:
+        big.free()
+        small.free()
+
+
+def test_static_pass_rejects_what_it_cannot_run():
+    lib = _library(3, st.ST_F32)
+    h = st.Histogram(100, 0.1, 10.0, "lin")
+    harr = (C.c_void_p * 1)(h.h)
+    o = _opts(2)
+    o.autotune = 1
+    rc = st.lib().st_batch_run_static(lib.h, C.byref(o), harr, 1, None, None)
+    assert rc == -1 and b"autotune" in st.lib().st_last_error()
+    # a library with a track for the exact path (a NaN point)
+    w = W.workload("cfg2")
+    x, y, z, e, off, _ = W.make_library(w, n_tracks=2)
+    x = x.copy()
+    x[3] = np.nan
+    lib2 = st.TrackLibrary(x, y, z, e, off, [st.SphericalGeometry(2.5e3, 10.0)], None, st.ST_F32)
+    assert lib2.info()["exact_tracks"] == 1
+    rc = st.lib().st_batch_run_static(lib2.h, C.byref(_opts(2)), harr, 1, None, None)
+    assert rc == -5 and b"exact" in st.lib().st_last_error()
+    # nothing was counted by the rejected calls
+    assert h.fetch().sum() == 0
+    h.free()
+    lib.free()
+    lib2.free()
