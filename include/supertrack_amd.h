@@ -180,7 +180,11 @@ int st_hist_edges(const st_hist *h, double *edges /* nbins + 1 */);
 int st_hist_begin_track(st_hist *h, const st_pairs *pairs);
 int st_hist_add(st_hist *h, const st_pairs *pairs);
 /* host_bins has nbins + 2 entries in ROOT order (0 underflow, 1..nbins,
- * nbins+1 overflow); the track's counts are added to entries 1..nbins. */
+ * nbins+1 overflow); the counts are added to entries 1..nbins and the device
+ * counters cleared.  host_bins NULL: no transfer; the track's counts stay
+ * summed in the device counters, the next st_hist_begin_track keeps them, and
+ * st_hist_fetch (or a later st_hist_end_track with host_bins) reads the
+ * total. */
 int st_hist_end_track(st_hist *h, double *host_bins);
 /* Copies the device counters accumulated by st_batch_run (u64, nbins entries)
  * to counts, optionally zeroing them. */

@@ -322,6 +322,9 @@ void batch_kernel(BatchArgs a)
     __shared__ long long s_seg;
     __shared__ int s_occ;
     __shared__ int s_range[2];
+    // entries each wave listed since the last flush, published at each
+    // occupancy check into alternating halves
+    __shared__ int s_wcnt[2][kBT / 64];
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -330,7 +333,8 @@ void batch_kernel(BatchArgs a)
     // conservation: energy words into and out of the table, entries created
     // and retired (a lost entry breaks one of the two equalities)
     unsigned long long bal_in = 0, bal_out = 0;
-    unsigned int n_created = 0, n_retired = 0;
+    // per wave (uniform): entries it listed from new points, entries retired
+    unsigned long long n_created = 0, n_retired = 0;
     for (int s = tid; s < SLOTS; s += kBT) {
         tkey[s] = kEmpty;
         tacc0[s] = 0;
@@ -341,10 +345,15 @@ void batch_kernel(BatchArgs a)
     __syncthreads();
     // upper bound on the counts added to any LDS bin since the last spill
     long long binned = 0;
+    // entries this wave listed since the last flush, and the half of s_wcnt
+    // the next check uses (both uniform)
+    int my_cnt = 0;
+    int par = 0;
 
     // Appends the slots that lanes newly occupied (slot >= 0) to the list:
     // one LDS atomic per wave reserves the positions.
-    auto append = [&](const int *slot, auto count_tag) {
+    // new_entries: the slots hold new entries (not survivors of a flush)
+    auto append = [&](const int *slot, auto count_tag, bool new_entries) {
         constexpr int kN = decltype(count_tag)::value;
         unsigned long long m[kN];
         int cnt = 0;
@@ -354,6 +363,8 @@ void batch_kernel(BatchArgs a)
             cnt += __popcll(m[j]);
         }
         if (cnt == 0) return;
+        my_cnt += cnt;
+        if (new_entries) n_created += (unsigned long long)cnt;
         int base = 0;
         if (lane == 0) base = atomicAdd(&s_occ, cnt);
         base = __shfl(base, 0, 64);
@@ -371,11 +382,13 @@ void batch_kernel(BatchArgs a)
     // re-inserted into the cleared table.  Threads walk the dense list of
     // used slots, so the work follows the entries, not the table size, and
     // the bin edges of two entries are loaded before either is compared.
-    auto flush = [&](unsigned long long bound) {
+    // after_check: called right after an occupancy check's barrier, which
+    // already ordered every wave's appends before this read of s_occ.
+    auto flush = [&](unsigned long long bound, bool after_check) {
         // raised issue priority while the workgroup's waves are tied together
         // by the flush barriers (measured: about 1 % on cfg 4)
         __builtin_amdgcn_s_setprio(2);
-        __syncthreads();
+        if (!after_check) __syncthreads();
         int n = uniform_i32(s_occ);
         if (n > kLP * kBT) {   // cannot happen (the bound above); reported, never read past the list
             if (tid == 0) atomicOr(&a.ctl->error, 2);
@@ -386,6 +399,8 @@ void batch_kernel(BatchArgs a)
         for (int r = 0; r < kLP; ++r) kk[r] = kEmpty;
 #pragma unroll
         for (int r0 = 0; r0 < kLP; r0 += 2) {
+            // rows past the list: nothing to read, convert or bin (n is uniform)
+            if (r0 * kBT >= n) break;
             double vv[2];
             bool em[2];
 #pragma unroll
@@ -407,10 +422,7 @@ void batch_kernel(BatchArgs a)
                     k0[r] = v0;
                     k1[r] = v1;
                     vv[u] = Acc<MODE>::value(v0, v1);
-                    if (em[u]) {
-                        bal_out += Acc<MODE>::balance(v0, v1);
-                        ++n_retired;
-                    }
+                    if (em[u]) bal_out += Acc<MODE>::balance(v0, v1);
                     if (em[u] && a.rec_cap > 0) {
                         const unsigned long long p = atomicAdd(&a.ctl->rec_count, 1ull);
                         if ((int64_t)p < a.rec_cap) {
@@ -455,6 +467,7 @@ void batch_kernel(BatchArgs a)
         }
         __syncthreads();
         if (tid == 0) s_occ = 0;
+        my_cnt = 0;
         __syncthreads();
         int placed[kLP];
 #pragma unroll
@@ -473,7 +486,17 @@ void batch_kernel(BatchArgs a)
                 if (kWords == 2) atomicAdd(&tacc1[h], k1[r]);
             }
         }
-        append(placed, std::integral_constant<int, kLP>());
+        // entries walked by this wave's lanes less those it placed again
+        {
+            unsigned long long walked = 0, kept = 0;
+#pragma unroll
+            for (int r = 0; r < kLP; ++r) {
+                walked += __popcll(__ballot(tid + r * kBT < n));
+                kept += __popcll(__ballot(placed[r] >= 0));
+            }
+            n_retired += walked - kept;
+        }
+        append(placed, std::integral_constant<int, kLP>(), false);
         __syncthreads();
         __builtin_amdgcn_s_setprio(0);
     };
@@ -621,10 +644,7 @@ void batch_kernel(BatchArgs a)
                         prev = atomicCAS(&tkey[h], kEmpty, key);
                     }
                     if (prev == kEmpty || prev == key) {
-                        if (prev == kEmpty) {
-                            fresh[j] = (int)h;
-                            ++n_created;
-                        }
+                        if (prev == kEmpty) fresh[j] = (int)h;
                         atomicAdd(&tacc0[h], (unsigned long long)v0);
                         if (kWords == 2) atomicAdd(&tacc1[h], (unsigned long long)v1);
                         bal_in += Acc<MODE>::balance((unsigned long long)v0, (unsigned long long)v1);
@@ -633,17 +653,23 @@ void batch_kernel(BatchArgs a)
                     }
                 }
             }
-            append(fresh, std::integral_constant<int, kPPT>());
+            append(fresh, std::integral_constant<int, kPPT>(), true);
             const int64_t next = c + (int64_t)kPPT * kBlk;
             if (++since < kCheck || next >= p_end) continue;
             since = 0;
-            // every wave reads the occupancy before any wave appends again:
-            // a wave that went on past the check could otherwise raise s_occ
-            // under a slower one, which would then flush alone
+            // Every wave must take the same decision.  Each publishes its own
+            // count into this check's half of s_wcnt and sums all of them
+            // after one barrier: a wave that runs on and appends more writes
+            // only its count for the next check, into the other half, so the
+            // counts read here cannot change under a slower wave (which would
+            // otherwise flush alone).  The sum equals s_occ at the barrier.
+            if (lane == 0) s_wcnt[par][tid >> 6] = my_cnt;
             __syncthreads();
-            const bool full = s_occ >= kFlushAt;
-            __syncthreads();
-            if (full) {
+            int occ = 0;
+#pragma unroll
+            for (int w = 0; w < kBT / 64; ++w) occ += s_wcnt[par][w];
+            par ^= 1;
+            if (occ >= kFlushAt) {
                 // every later point lies in a layer >= the next point's exact
                 // layer index (z order), and a key's layer (key / N^2) is never
                 // below its point's layer: entries with smaller keys are complete
@@ -651,10 +677,10 @@ void batch_kernel(BatchArgs a)
                 double lb = nearest_index(zs - g.gso, g.D, g.invD);
                 if (!(lb > 0.0)) lb = 0.0;
                 const unsigned long long bound = (unsigned long long)lb * g.N2;
-                flush(bound);
+                flush(bound, true);
             }
         }
-        flush(kEmpty);
+        flush(kEmpty, false);
         if (a.hist_lds_words && binned > 0x40000000ll) {
             // keep the 32-bit LDS counters far from overflow
 #pragma unroll
@@ -674,15 +700,19 @@ void batch_kernel(BatchArgs a)
         }
     }
     {
-        unsigned long long c[kStats] = {n_scored, n_flagged, n_in, bal_in, bal_out, n_created, n_retired};
+        // per-lane counters summed over the wave; n_created / n_retired are
+        // already per wave
+        unsigned long long c[5] = {n_scored, n_flagged, n_in, bal_in, bal_out};
 #pragma unroll
         for (int d = 32; d >= 1; d >>= 1) {
 #pragma unroll
-            for (int k = 0; k < kStats; ++k) c[k] += __shfl_xor(c[k], d, 64);
+            for (int k = 0; k < 5; ++k) c[k] += __shfl_xor(c[k], d, 64);
         }
         if (lane == 0) {
 #pragma unroll
-            for (int k = 0; k < kStats; ++k) atomicAdd(&a.ctl->stats[k], c[k]);
+            for (int k = 0; k < 5; ++k) atomicAdd(&a.ctl->stats[k], c[k]);
+            atomicAdd(&a.ctl->stats[5], n_created);
+            atomicAdd(&a.ctl->stats[6], n_retired);
         }
     }
     __syncthreads();
@@ -858,6 +888,7 @@ BatchArgs make_args(const st_library *L, const st_batch_opts &o, st_hist *const 
     a.n_hists = n_hists;
     int off = 0;
     for (int h = 0; h < n_hists; ++h) {
+        hists[h]->bins_zero = false;   // the pass counts into them
         a.hist[h].edges = hists[h]->edges_dev;
         a.hist[h].bins = (unsigned long long *)hists[h]->bins_dev;
         a.hist[h].guess = hists[h]->guess;

@@ -41,35 +41,38 @@ __device__ __forceinline__ double nearest_index(double a, double D, double invD)
 // ---------------------------------------------------------------------------
 // per-track kernels
 // ---------------------------------------------------------------------------
-// ShiftTrackZOrigin (VoxelConstrainedSphereMethod.cu:138-154).
-__global__ void zshift_kernel(double *__restrict__ z, int64_t n, double half_length)
+// Per-track set-up in one launch: ShiftTrackZOrigin's z += H'
+// (VoxelConstrainedSphereMethod.cu:138-154), the lineal energy e / ((2/3) D)
+// of each point (:302,322), which depends only on the point and so is taken
+// once per track instead of once per oversample (the same IEEE value), and
+// GenerateRandomXYShift's shift table (:61-83) with Philox.
+struct SetupArgs {
+    double *z;
+    const double *e;
+    double *lin;
+    int64_t n;
+    double half_length, denominator, length;
+    double *shifts;   // 3 per oversample
+    int n_os;
+    uint64_t seed;
+    uint32_t track_index;
+    int shift_xy, shift_z;
+};
+
+__global__ void track_setup_kernel(SetupArgs a)
 {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-        z[i] = z[i] + half_length;
-}
-
-// e / ((2/3) D) (VoxelConstrainedSphereMethod.cu:302,322) depends only on the
-// point, so it is computed once per track instead of once per oversample.
-__global__ void lineal_kernel(const double *__restrict__ e, double *__restrict__ lin, int64_t n,
-                              double denominator)
-{
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-        lin[i] = e[i] / denominator;
-}
-
-// GenerateRandomXYShift (VoxelConstrainedSphereMethod.cu:61-83) with Philox.
-__global__ void shift_table_kernel(double *__restrict__ shifts, int n_os, uint64_t seed,
-                                   uint32_t track_index, int shift_xy, int shift_z, double length,
-                                   double half_length)
-{
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n_os) return;
-    const Shift s = shift_for(seed, track_index, (uint32_t)k, shift_xy, shift_z, length, half_length);
-    shifts[3 * k + 0] = s.x;
-    shifts[3 * k + 1] = s.y;
-    shifts[3 * k + 2] = s.z;
+    const int64_t first = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (int64_t i = first; i < a.n; i += stride) {
+        a.z[i] = a.z[i] + a.half_length;
+        a.lin[i] = a.e[i] / a.denominator;
+    }
+    for (int64_t k = first; k < a.n_os; k += stride) {
+        const Shift s = shift_for(a.seed, a.track_index, (uint32_t)k, a.shift_xy, a.shift_z, a.length, a.half_length);
+        a.shifts[3 * k + 0] = s.x;
+        a.shifts[3 * k + 1] = s.y;
+        a.shifts[3 * k + 2] = s.z;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -982,6 +985,7 @@ int hist_add_pairs(st_hist *h, const st_pairs *pairs, hipStream_t stream)
     ha.edges = h->edges_dev;
     ha.guess = h->guess;
     ha.bins = (unsigned long long *)h->bins_dev;
+    h->bins_zero = false;
     // small inputs (decided on the device) take the one-workgroup LDS sort
     hist_lds_kernel<<<1, kSortThreads, kLdsBytes, stream>>>(ha);
     ST_HIP_TRY(hipGetLastError());
@@ -1072,6 +1076,7 @@ int queue_flush(st_vcs *m)
                     ha.guess = h->guess;
                     ha.bins = (unsigned long long *)h->bins_dev;
                     if (std::find(consumers.begin(), consumers.end(), h) == consumers.end()) consumers.push_back(h);
+                    h->bins_zero = false;
                 }
                 queued_track_kernel<<<a.n_os, kSortThreads, kLdsBytes, m->stream>>>(a);
                 if (hipGetLastError() != hipSuccess) rc = set_error(ST_EHIP, "queued_track_kernel launch failed");
@@ -1322,16 +1327,25 @@ int st_vcs_begin_track(st_vcs *m, st_track *track, uint64_t seed, uint32_t track
         if (rc) return rc;
         ST_HIP_TRY(hipMemsetAsync(m->status, 0, (size_t)tiles * sizeof(unsigned long long), m->stream));
     }
-    // ShiftTrackZOrigin, once per track (VoxelConstrainedSphereMethod.cu:45)
-    if (n > 0) {
-        zshift_kernel<<<grid_for(n, 256, 4096), 256, 0, m->stream>>>(track->z, n, m->g.half_length);
-        lineal_kernel<<<grid_for(n, 256, 4096), 256, 0, m->stream>>>(track->edep, m->lineal, n,
-                                                                       (2. / 3.) * m->g.diameter);
+    // ShiftTrackZOrigin, once per track (VoxelConstrainedSphereMethod.cu:45),
+    // the lineal energies and the shift table
+    if (n > 0 || n_oversamples > 0) {
+        SetupArgs sa;
+        sa.z = track->z;
+        sa.e = track->edep;
+        sa.lin = m->lineal;
+        sa.n = n;
+        sa.half_length = m->g.half_length;
+        sa.denominator = (2. / 3.) * m->g.diameter;
+        sa.length = m->g.length;
+        sa.shifts = m->shifts;
+        sa.n_os = n_oversamples;
+        sa.seed = seed;
+        sa.track_index = track_index;
+        sa.shift_xy = m->shift_xy;
+        sa.shift_z = m->shift_z;
+        track_setup_kernel<<<grid_for(std::max<int64_t>(n, n_oversamples), 256, 4096), 256, 0, m->stream>>>(sa);
     }
-    if (n_oversamples > 0)
-        shift_table_kernel<<<grid_for(n_oversamples, 256), 256, 0, m->stream>>>(
-            m->shifts, n_oversamples, seed, track_index, m->shift_xy, m->shift_z, m->g.length,
-            m->g.half_length);
     ST_HIP_TRY(hipGetLastError());
     m->n = n;
     m->n_oversamples = n_oversamples;
@@ -1505,7 +1519,11 @@ int st_hist_set_stream(st_hist *h, void *hip_stream)
     if (!h) return set_error(ST_EINVAL, "st_hist_set_stream: null histogram");
     int rc = queue_flush_producers(h);
     if (rc) return rc;
-    h->stream = hip_stream ? (hipStream_t)hip_stream : default_stream();
+    hipStream_t next = hip_stream ? (hipStream_t)hip_stream : default_stream();
+    // work still queued on the old stream (a clear of the counters) is done
+    // before the new stream can touch them
+    if (next != h->stream && h->stream) ST_HIP_TRY(hipStreamSynchronize(h->stream));
+    h->stream = next;
     return ST_OK;
 }
 
@@ -1538,10 +1556,15 @@ int st_hist_begin_track(st_hist *h, const st_pairs *pairs)
         rc = ensure_sort_buffers(h, h->capacity);
         if (rc) return rc;
     }
-    // the per-track counters start at zero (Histogram.cu:139-140)
-    zero_u64_kernel<<<grid_for(h->nbins, 256, 1024), 256, 0, h->stream>>>(
-        (unsigned long long *)h->bins_dev, h->nbins);
-    ST_HIP_TRY(hipGetLastError());
+    // the per-track counters start at zero (Histogram.cu:139-140); they
+    // already are unless something was counted since they were cleared, and
+    // they keep the counts carried from tracks ended without a host copy
+    if (!h->bins_zero && !h->carry) {
+        zero_u64_kernel<<<grid_for(h->nbins, 256, 1024), 256, 0, h->stream>>>(
+            (unsigned long long *)h->bins_dev, h->nbins);
+        ST_HIP_TRY(hipGetLastError());
+        h->bins_zero = true;
+    }
     h->in_track = true;
     return ST_OK;
 }
@@ -1576,14 +1599,21 @@ int st_hist_end_track(st_hist *h, double *host_bins)
     if (!h->in_track) return set_error(ST_ESTATE, "st_hist_end_track before st_hist_begin_track");
     int rc = queue_flush_producers(h);
     if (rc) return rc;
+    if (!host_bins) {
+        // the counts stay summed in the device counters
+        h->carry = true;
+        h->in_track = false;
+        return ST_OK;
+    }
     ST_HIP_TRY(hipMemcpyAsync(h->host_counts.data(), h->bins_dev, (size_t)h->nbins * sizeof(uint64_t),
                               hipMemcpyDeviceToHost, h->stream));
     ST_HIP_TRY(hipStreamSynchronize(h->stream));
-    if (host_bins)
-        for (int i = 0; i < h->nbins; ++i) host_bins[i + 1] += (double)h->host_counts[i];
+    for (int i = 0; i < h->nbins; ++i) host_bins[i + 1] += (double)h->host_counts[i];
     zero_u64_kernel<<<grid_for(h->nbins, 256, 1024), 256, 0, h->stream>>>(
         (unsigned long long *)h->bins_dev, h->nbins);
     ST_HIP_TRY(hipGetLastError());
+    h->bins_zero = true;
+    h->carry = false;
     h->in_track = false;
     return ST_OK;
 }
@@ -1601,6 +1631,8 @@ int st_hist_fetch(st_hist *h, uint64_t *counts, int32_t reset)
             (unsigned long long *)h->bins_dev, h->nbins);
         ST_HIP_TRY(hipGetLastError());
         ST_HIP_TRY(hipStreamSynchronize(h->stream));
+        h->bins_zero = true;
+        h->carry = false;
     }
     return ST_OK;
 }

@@ -72,6 +72,12 @@ struct st_hist {
     std::vector<uint64_t> host_counts;
     // methods with queued oversamples whose pairs this histogram consumed
     std::vector<st_vcs *> producers;
+    // the device counters are zero once the work queued on `stream` is done
+    // (nothing has been counted since they were last cleared)
+    bool bins_zero = true;
+    // tracks were ended without a host copy: the counters carry their counts
+    // into the next track (st_hist_end_track with host_bins NULL)
+    bool carry = false;
 };
 
 // VoxelConstrainedSphereMethod object (VoxelConstrainedSphereMethod.hh:43-75).

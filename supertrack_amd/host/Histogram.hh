@@ -62,11 +62,19 @@ public:
     void SetStream(void *stream);
     void Print();
 
-    Hist1D GetCPUHistogram() { return _CPU_histogram; }
+    // The CPU histogram, after adding the counts still held on the device
+    // (FreeTrackProcess leaves each track's counts there: one transfer at the
+    // end instead of one per track; the sums are exact integers either way).
+    Hist1D GetCPUHistogram()
+    {
+        AccumulateBatched();
+        return _CPU_histogram;
+    }
     static Hist1D ReduceVector(std::vector<Hist1D> &toReduce);
 
     st_hist *Handle() { return _hist; }
-    // Adds the device counters accumulated by batched runs to the CPU copy.
+    // Adds the device counters (batched runs, tracks ended by FreeTrackProcess)
+    // to the CPU copy and clears them.
     void AccumulateBatched();
 
 private:

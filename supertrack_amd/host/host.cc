@@ -499,11 +499,13 @@ void Histogram::SetStream(void *stream)
     check(st_hist_set_stream(_hist, stream), "Histogram::SetStream");
 }
 
+// AccumulateCPUHistogram (Histogram.cu:177-186) adds each track's counts to the
+// CPU histogram; here they stay summed in the device counters until
+// GetCPUHistogram / Print reads them, which gives the same integer totals
+// without a device-to-host transfer and synchronisation per track.
 void Histogram::FreeTrackProcess()
 {
-    std::vector<double> add(_nbins + 2, 0.0);
-    check(st_hist_end_track(_hist, add.data()), "Histogram::FreeTrackProcess");
-    for (int i = 1; i <= _nbins; ++i) _CPU_histogram.AddBinContent(i, add[i]);
+    check(st_hist_end_track(_hist, nullptr), "Histogram::FreeTrackProcess");
 }
 
 void Histogram::AccumulateBatched()
@@ -515,6 +517,7 @@ void Histogram::AccumulateBatched()
 
 void Histogram::Print()
 {
+    AccumulateBatched();
     double total = 0;
     for (int i = 1; i <= _nbins; ++i) {
         std::cout << "Bin: " << _CPU_histogram.GetBinLowEdge(i) << " Counts: " << _CPU_histogram.GetBinContent(i)
@@ -622,7 +625,8 @@ void SuperTrackManager::RunPerTrack()
                 method->ProcessTrack(track, edepsInTarget);
                 histogram.SortReduceAndAddToHistogram(edepsInTarget);
             }
-            if (hipStreamSynchronize(stream) != hipSuccess) fatal("hipStreamSynchronize failed");
+            // cudaDeviceSynchronize of the reference's loop: FreeTrackProcess
+            // synchronises the stream (all of this worker's work is on it)
             method->FreeTrackProcess();
             histogram.FreeTrackProcess();
         }

@@ -57,6 +57,13 @@ int main(int argc, char **argv)
     SuperTrackManager &manager = SuperTrackManager::GetInstance();
     manager.AddSimulationMethod("VoxelConstrainedSphere", &VoxelConstrainedSphereMethod::Construct);
     manager.Initialize(new INIReader(path));
+    // the HIP runtime and the device are set up before the run clock starts:
+    // "Run time" covers the run, not the process start
+    int devices = 0;
+    if (st_device_count(&devices) != ST_OK || devices < 1) {
+        std::cerr << "No GPU: " << st_last_error() << std::endl;
+        return 1;
+    }
     const auto t0 = std::chrono::steady_clock::now();
     manager.Run();
     const auto t1 = std::chrono::steady_clock::now();
