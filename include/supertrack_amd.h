@@ -243,6 +243,9 @@ typedef struct {
                                      3 (ST_F64), capped by occupancy           */
     int32_t points_per_thread;    /* 1 or 2 per iteration; 0 = 1 (ST_F32 at five
                                      workgroups per CU, else 2), 2 (ST_F64)    */
+    int32_t hist_copies;          /* LDS copies of the histograms per workgroup
+                                     (1, 2 or 4; wave w counts into copy
+                                     w % copies); 0 = 1 or autotuned          */
 } st_batch_opts;
 
 typedef struct st_batch st_batch;
@@ -257,9 +260,10 @@ int st_batch_sync(st_batch *b);
 /* Device time of the last st_batch_run in ms: ms2[0] the fused kernel
  * (with its reset), ms2[1] the exact per-segment path. */
 int st_batch_times(st_batch *b, float *ms2);
-/* Launch parameters in use: block_threads, table_slots, blocks_per_cu,
- * points_per_thread. */
-int st_batch_params(const st_batch *b, int32_t *params4);
+/* Launch parameters in use (5 entries): block_threads, table_slots,
+ * blocks_per_cu, points_per_thread, hist_copies (0: the histograms count
+ * straight into the device counters, too large for LDS). */
+int st_batch_params(const st_batch *b, int32_t *params5);
 /* Counts of the last st_batch_run's fused kernel: stats3[0] points scored
  * (points of the in-voxel block ranges of the segments that pass the x/y
  * cull), stats3[1] of those whose FP32 decision was within the error bound

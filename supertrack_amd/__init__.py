@@ -402,7 +402,8 @@ class BatchRun:
     def __init__(self, library: TrackLibrary, histograms: Sequence[Histogram], n_oversamples: int,
                  seed: int, shift_xy: bool = True, shift_z: bool = False, track_index_base: int = 0,
                  use_graph: bool = True, autotune: bool = False, block_threads: int = 0,
-                 table_slots: int = 0, blocks_per_cu: int = 0, points_per_thread: int = 0):
+                 table_slots: int = 0, blocks_per_cu: int = 0, points_per_thread: int = 0,
+                 hist_copies: int = 0):
         o = _capi.BatchOpts()
         o.n_oversamples = n_oversamples
         o.shift_xy = int(bool(shift_xy))
@@ -415,6 +416,7 @@ class BatchRun:
         o.table_slots = table_slots
         o.blocks_per_cu = blocks_per_cu
         o.points_per_thread = points_per_thread
+        o.hist_copies = hist_copies
         hs = (C.c_void_p * len(histograms))(*[h.h for h in histograms])
         self.h = C.c_void_p()
         self.library = library
@@ -449,10 +451,10 @@ class BatchRun:
                 "entries_retired": int(c[3])}
 
     def params(self):
-        p = np.zeros(4, np.int32)
+        p = np.zeros(5, np.int32)
         call("st_batch_params", self.h, p.ctypes.data_as(_capi.c_int32_p))
         return {"block_threads": int(p[0]), "table_slots": int(p[1]), "blocks_per_cu": int(p[2]),
-                "points_per_thread": int(p[3])}
+                "points_per_thread": int(p[3]), "hist_copies": int(p[4])}
 
     def record(self, capacity: int):
         call("st_batch_record", self.h, C.c_int64(capacity))

@@ -50,7 +50,7 @@ class BatchOpts(C.Structure):
     _fields_ = [("n_oversamples", C.c_int32), ("shift_xy", C.c_int32), ("shift_z", C.c_int32),
                 ("track_index_base", C.c_uint32), ("seed", C.c_uint64), ("use_graph", C.c_int32),
                 ("autotune", C.c_int32), ("block_threads", C.c_int32), ("table_slots", C.c_int32),
-                ("blocks_per_cu", C.c_int32), ("points_per_thread", C.c_int32)]
+                ("blocks_per_cu", C.c_int32), ("points_per_thread", C.c_int32), ("hist_copies", C.c_int32)]
 
 
 class SynthSpec(C.Structure):

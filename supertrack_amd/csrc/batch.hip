@@ -117,6 +117,7 @@ struct BatchArgs {
     HistDev hist[kMaxHists];
     int n_hists;
     int hist_lds_words;     // 0: bins go straight to the device counters
+    int hist_copies;        // LDS copies of the histograms (wave w counts into copy w % copies)
 
     // recording (test hook)
     int64_t rec_cap;
@@ -340,7 +341,10 @@ void batch_kernel(BatchArgs a)
         tacc0[s] = 0;
         if (kWords == 2) tacc1[s] = 0;
     }
-    for (int w = tid; w < a.hist_lds_words; w += kBT) lhist[w] = 0;
+    for (int w = tid; w < a.hist_lds_words * a.hist_copies; w += kBT) lhist[w] = 0;
+    // this wave's copy of the privatised histograms (more copies, fewer
+    // waves contending for one hot bin's LDS word)
+    unsigned int *const lhist_w = lhist + ((tid >> 6) % a.hist_copies) * a.hist_lds_words;
     if (tid == 0) s_occ = 0;
     __syncthreads();
     // upper bound on the counts added to any LDS bin since the last spill
@@ -459,7 +463,7 @@ void batch_kernel(BatchArgs a)
                     int b = bb[u];
                     if (!(v >= lo[u] && v < hi[u])) b = bin_fast(hd.edges, hd.guess, v);
                     if (b >= 0) {
-                        if (a.hist_lds_words) atomicAdd(&lhist[hd.lds_offset + b], 1u);
+                        if (a.hist_lds_words) atomicAdd(&lhist_w[hd.lds_offset + b], 1u);
                         else atomicAdd(&hd.bins[b], 1ull);
                     }
                 }
@@ -688,11 +692,13 @@ void batch_kernel(BatchArgs a)
                 if (h >= a.n_hists) continue;
                 const int nb = a.hist[h].guess.nbins;
                 for (int b = tid; b < nb; b += kBT) {
-                    const unsigned int v = lhist[a.hist[h].lds_offset + b];
-                    if (v) {
-                        atomicAdd(&a.hist[h].bins[b], (unsigned long long)v);
-                        lhist[a.hist[h].lds_offset + b] = 0;
+                    unsigned long long v = 0;
+                    for (int cpy = 0; cpy < a.hist_copies; ++cpy) {
+                        unsigned int *w = &lhist[cpy * a.hist_lds_words + a.hist[h].lds_offset + b];
+                        v += *w;
+                        *w = 0;
                     }
+                    if (v) atomicAdd(&a.hist[h].bins[b], v);
                 }
             }
             __syncthreads();
@@ -722,8 +728,9 @@ void batch_kernel(BatchArgs a)
         if (h >= a.n_hists) continue;
         const int nb = a.hist[h].guess.nbins;
         for (int b = tid; b < nb; b += kBT) {
-            const unsigned int v = lhist[a.hist[h].lds_offset + b];
-            if (v) atomicAdd(&a.hist[h].bins[b], (unsigned long long)v);
+            unsigned long long v = 0;
+            for (int cpy = 0; cpy < a.hist_copies; ++cpy) v += lhist[cpy * a.hist_lds_words + a.hist[h].lds_offset + b];
+            if (v) atomicAdd(&a.hist[h].bins[b], v);
         }
     }
 }
@@ -810,6 +817,7 @@ int dev_upload(T **dst, const T *src, size_t n, int64_t *bytes)
 struct Launch {
     int slots = 1536, blocks_per_cu = 1, ppt = 1, grid = 0;
     bool lds_hist = true;     // histograms privatised in LDS (else device atomics)
+    int hist_copies = 1;      // LDS copies of the privatised histograms
     size_t lds = 0;
 };
 
@@ -896,6 +904,7 @@ BatchArgs make_args(const st_library *L, const st_batch_opts &o, st_hist *const 
         off += hists[h]->nbins;
     }
     a.hist_lds_words = ln.lds_hist ? off : 0;
+    a.hist_copies = ln.lds_hist ? ln.hist_copies : 1;
     return a;
 }
 
@@ -922,16 +931,22 @@ int launch_fast(st_batch *b)
     return launch_pass(b->lib->precision, a, b->ln, b->stream);
 }
 
-// Launch shape for (table slots, workgroups per CU, points per thread),
-// capped by what the kernel's registers and LDS allow.
-int configure_launch(int precision, int hist_words, int slots, int bpc, int ppt, Launch *out)
+// Launch shape for (table slots, workgroups per CU, points per thread, LDS
+// copies of the histograms), capped by what the kernel's registers and LDS
+// allow.
+int configure_launch(int precision, int hist_words, int slots, int bpc, int ppt, int copies, Launch *out)
 {
     Launch ln;
     ln.ppt = ppt == 2 ? 2 : 1;
+    if (copies != 1 && copies != 2 && copies != 4)
+        return set_error(ST_EINVAL, "batched run: hist_copies must be 1, 2 or 4");
     // privatise the histograms in LDS when they fit next to the table at two
-    // workgroups per CU; otherwise count straight into the device bins
-    ln.lds_hist = lds_bytes(precision, slots, hist_words) <= 80 * 1024;
-    const size_t lds = lds_bytes(precision, slots, ln.lds_hist ? hist_words : 0);
+    // workgroups per CU; otherwise count straight into the device bins (one copy)
+    ln.lds_hist = lds_bytes(precision, slots, hist_words * copies) <= 80 * 1024;
+    if (!ln.lds_hist && copies > 1)
+        return set_error(ST_ERANGE, "batched run: %d LDS copies of %d histogram bins do not fit", copies, hist_words);
+    ln.hist_copies = copies;
+    const size_t lds = lds_bytes(precision, slots, ln.lds_hist ? hist_words * copies : 0);
     if (lds > 160 * 1024)
         return set_error(ST_ERANGE, "batched kernel needs %zu B of LDS (limit 160 KiB)", lds);
     const void *k = pick_kernel(precision, slots, ln.ppt);
@@ -948,9 +963,9 @@ int configure_launch(int precision, int hist_words, int slots, int bpc, int ppt,
     return ST_OK;
 }
 
-int configure(st_batch *b, int slots, int bpc, int ppt)
+int configure(st_batch *b, int slots, int bpc, int ppt, int copies)
 {
-    return configure_launch(b->lib->precision, b->hist_words, slots, bpc, ppt, &b->ln);
+    return configure_launch(b->lib->precision, b->hist_words, slots, bpc, ppt, copies, &b->ln);
 }
 
 // The launch shape used unless the options name one or ask for tuning: every
@@ -967,7 +982,8 @@ int default_launch(int precision, int hist_words, const st_batch_opts &o, Launch
     const int bpc = o.blocks_per_cu ? o.blocks_per_cu : (f32 ? 5 : 3);
     const int ppt = o.points_per_thread ? o.points_per_thread : (f32 ? 1 : 2);
     if (ppt != 1 && ppt != 2) return set_error(ST_EINVAL, "batched run: points_per_thread must be 1 or 2");
-    int rc = configure_launch(precision, hist_words, slots, bpc, ppt, out);
+    const int copies = o.hist_copies ? o.hist_copies : 1;
+    int rc = configure_launch(precision, hist_words, slots, bpc, ppt, copies, out);
     if (rc) return rc;
     // FP32 with fewer than five workgroups per CU (large privatised
     // histograms): a 1280-slot table if it fits more workgroups, else two
@@ -976,14 +992,14 @@ int default_launch(int precision, int hist_words, const st_batch_opts &o, Launch
         const int fit1536 = out->blocks_per_cu;
         if (!o.table_slots) {
             Launch alt;
-            if (configure_launch(precision, hist_words, 1280, bpc, 1, &alt) == ST_OK &&
+            if (configure_launch(precision, hist_words, 1280, bpc, 1, copies, &alt) == ST_OK &&
                 alt.blocks_per_cu > fit1536) {
                 *out = alt;
                 return ST_OK;
             }
             clear_error();
         }
-        return configure_launch(precision, hist_words, slots, bpc, 2, out);
+        return configure_launch(precision, hist_words, slots, bpc, 2, copies, out);
     }
     return ST_OK;
 }
@@ -1057,12 +1073,14 @@ int autotune(st_batch *b)
     ST_HIP_TRY(hipStreamSynchronize(b->stream));
     const int cand_slots[] = {1280, 1536, 2048, 4096};
     const int cand_bpc[] = {2, 3, 4, 5, 6, 8};
+    const int cand_copies[] = {1, 2, 4};
     float best = 1e30f;
-    int best_s = b->ln.slots, best_b = b->ln.blocks_per_cu, best_p = b->ln.ppt;
+    int best_s = b->ln.slots, best_b = b->ln.blocks_per_cu, best_p = b->ln.ppt, best_c = b->ln.hist_copies;
     for (int s : cand_slots)
         for (int pp = 1; pp <= 2; ++pp)
-            for (int bp : cand_bpc) {
-                if (configure(b, s, bp, pp) != ST_OK) continue;
+            for (int bp : cand_bpc)
+                for (int cp : cand_copies) {
+                if (configure(b, s, bp, pp, cp) != ST_OK) continue;
                 if (b->ln.blocks_per_cu != bp) continue;
                 ST_HIP_TRY(hipEventRecord(ev.e[0], b->stream));
                 int rc = launch_fast(b);
@@ -1076,6 +1094,7 @@ int autotune(st_batch *b)
                     best_s = s;
                     best_b = bp;
                     best_p = pp;
+                    best_c = cp;
                 }
             }
     for (size_t i = 0; i < b->hists.size(); ++i)
@@ -1083,7 +1102,7 @@ int autotune(st_batch *b)
                                   hipMemcpyHostToDevice, b->stream));
     ST_HIP_TRY(hipStreamSynchronize(b->stream));
     clear_error();   // candidates that did not fit set an error string
-    return configure(b, best_s, best_b, best_p);
+    return configure(b, best_s, best_b, best_p, best_c);
 }
 
 // ---------------------------------------------------------------------------
@@ -1526,6 +1545,7 @@ int st_batch_params(const st_batch *b, int32_t *p)
     p[1] = b->ln.slots;
     p[2] = b->ln.blocks_per_cu;
     p[3] = b->ln.ppt;
+    p[4] = b->ln.lds_hist ? b->ln.hist_copies : 0;
     return ST_OK;
 }
 

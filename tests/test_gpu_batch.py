@@ -557,3 +557,26 @@ def test_batch_bench_sample_against_oracle(name, stride):
               f"1e-6 of an edge (oracle {secs:.1f} s on {threads} threads)", flush=True)
         assert diff <= 2 * near[h], (h, diff, near[h])
     assert total > 10**7
+
+
+def test_batch_histogram_copies_are_invariant():
+    """1, 2 and 4 LDS copies of the privatised histograms per workgroup give
+    identical counts (one log histogram; and lin 3000 + log 1000 together,
+    where four copies do not fit next to the table and are refused)."""
+    w = W.workload("cfg3")
+    spec = dict(w["synth"], n_max=80000)
+    x, y, z, e, off = st.synth_library(spec, 20, 300)
+    for hists, copies in (([("log", 1000, 1e-2, 1e3)], (1, 2, 4)),
+                          ([("lin", 3000, 0.01, 300.0), ("log", 1000, 1e-2, 1e3)], (1, 2))):
+        results = []
+        for c in copies:
+            counts, _, info, _ = gpu_run(x, y, z, e, off, [(2.5e3, 10.0)], None, 6, 8, True, True, st.ST_F32,
+                                         hists, hist_copies=c)
+            results.append(counts)
+        for r in results[1:]:
+            for a, b in zip(r, results[0]):
+                np.testing.assert_array_equal(a, b)
+        assert sum(int(a.sum()) for a in results[0]) > 1000
+    with pytest.raises(st.SuperTrackError, match="copies"):
+        gpu_run(x, y, z, e, off, [(2.5e3, 10.0)], None, 2, 8, True, True, st.ST_F32,
+                [("lin", 3000, 0.01, 300.0), ("log", 1000, 1e-2, 1e3)], hist_copies=4)
