@@ -48,7 +48,6 @@ namespace st {
 namespace {
 
 constexpr int kBT = 256;                    // threads per workgroup
-constexpr unsigned long long kEmpty = ~0ull;
 constexpr int kMaxGeoms = 16;
 constexpr int kMaxHists = 2;
 
@@ -91,7 +90,9 @@ constexpr int kStats = 7;
 struct BatchCtl {
     unsigned long long ticket;
     unsigned long long rec_count;
-    int error;   // 1: a table filled up, 2: a flush found more entries than its list bound
+    // 1: a table filled up, 2: a flush found more entries than its list bound,
+    // 4: an in-sphere point's key outside its segment's key box
+    int error;
     int pad;
     unsigned long long stats[kStats];
 };
@@ -122,7 +123,8 @@ struct BatchArgs {
     // recording (test hook)
     int64_t rec_cap;
     int64_t *rec_seg;
-    unsigned long long *rec_key;
+    unsigned long long *rec_key;   // the local key
+    unsigned long long *rec_box;   // 3 words per record: kx0 | ky0 << 32, kz0 | nx << 32, ny
     double *rec_sum;
 };
 
@@ -131,11 +133,32 @@ __device__ __forceinline__ int uniform_i32(int v)
     return __builtin_amdgcn_readfirstlane(v);
 }
 
+__device__ __forceinline__ unsigned uniform_u32(unsigned v)
+{
+    return (unsigned)__builtin_amdgcn_readfirstlane((int)v);
+}
+
 __device__ __forceinline__ long long uniform_i64(long long v)
 {
     const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)v);
     const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)((unsigned long long)v >> 32));
     return (long long)(((unsigned long long)hi << 32) | lo);
+}
+
+// A record through the scalar cache (read-only during the kernel, address
+// uniform across the workgroup): its fields land in scalar registers.
+template <typename T>
+__device__ __forceinline__ T load_uniform(const T *p)
+{
+    static_assert(sizeof(T) % 8 == 0 && alignof(T) >= 8, "records of 8-byte words");
+    typedef const __attribute__((address_space(4))) unsigned long long *ConstPtr;
+    const ConstPtr q = (ConstPtr)(const void *)p;
+    unsigned long long w[sizeof(T) / 8];
+#pragma unroll
+    for (size_t i = 0; i < sizeof(T) / 8; ++i) w[i] = q[i];
+    T r;
+    __builtin_memcpy(&r, w, sizeof(T));
+    return r;
 }
 
 // Block record through the scalar cache (read-only during the kernel, index
@@ -154,12 +177,31 @@ __device__ __forceinline__ BlockF32 load_block(const BlockF32 *p, int64_t idx)
     return r;
 }
 
-// key = kx + ky * N + kz * N^2 (mod 2^64, as the reference's unsigned sum)
+// the reference's key: kx + ky * N + kz * N^2 (mod 2^64, its unsigned sum)
 __device__ __forceinline__ unsigned long long lattice_key(unsigned kx, unsigned ky, unsigned kz, unsigned N)
 {
     const unsigned long long kyz = (unsigned long long)kz * N + ky;
     return kyz * N + kx;
 }
+
+// The sphere table holds 32-bit keys local to a slab of the segment's box:
+//   key = ((kz - kz0) * ny + (ky - ky0)) * nx + (kx - kx0)
+// over the box [kx0, kx0 + nx) x [ky0, ky0 + ny) x [kz0, kz0 + nz) of sphere
+// indices that the segment's points can reach (from the track's bounds and
+// the shift), with nx * ny * nz <= 2^32 - 2 so that every key is below
+// kEmpty32.  Within a segment the map is one-to-one and keeps the layer-major
+// order of the reference's key, so "key below the first layer still to come"
+// is the same test in both.  A segment whose box exceeds 2^32 - 2 spheres is
+// scored in slabs of layers, each with its own box and a full flush at its end.
+constexpr unsigned kEmpty32 = 0xFFFFFFFFu;
+constexpr unsigned long long kMaxKeys = 0xFFFFFFFEull;   // keys per slab box
+
+struct SlabBox {
+    unsigned kx0, ky0, kz0;   // global index of local (0, 0, 0)
+    unsigned nx, ny, nz;      // extent (nz: this slab's layers)
+    unsigned nxy;             // nx * ny
+};
+
 
 // ---------------------------------------------------------------------------
 // exact FP64 membership (the reference's operations, rounding as written)
@@ -173,8 +215,11 @@ __device__ __forceinline__ double nearest_index(double a, double D, double invD)
     return k;
 }
 
+// On success kx, ky, kz are the reference's sphere indices, in [0, N]: an
+// index equals N only for a point within rounding of the voxel's upper face.
 __device__ __forceinline__ bool exact_member(double x, double y, double zp, const GeomDev &g,
-                                             double sx, double sy, double sz, unsigned long long &key)
+                                             double sx, double sy, double sz, unsigned &ix, unsigned &iy,
+                                             unsigned &iz)
 {
     const double xs = x + sx;
     const double ys = y + sy;
@@ -190,7 +235,9 @@ __device__ __forceinline__ bool exact_member(double x, double y, double zp, cons
     const double dist = ((dx * dx) + (dy * dy)) + (dz * dz);
     if (!(dist <= g.r2)) return false;
     // in-voxel indices lie in [0, N] with N < 2^31
-    key = lattice_key((unsigned)(int)kx, (unsigned)(int)ky, (unsigned)(int)kz, (unsigned)g.N);
+    ix = (unsigned)(int)kx;
+    iy = (unsigned)(int)ky;
+    iz = (unsigned)(int)kz;
     return true;
 }
 
@@ -235,13 +282,12 @@ struct Acc<ST_F32> {
     __device__ static unsigned long long balance(unsigned long long a0, unsigned long long) { return a0; }
 };
 
-// Home slot in [0, SLOTS) of a 64-bit key: 32-bit multiplicative hash
-// scaled to the table size.
+// Home slot in [0, SLOTS) of a key: multiplicative (Fibonacci) hash scaled
+// to the table size.
 template <int SLOTS>
-__device__ __forceinline__ unsigned int slot_hash(unsigned long long key)
+__device__ __forceinline__ unsigned int slot_hash(unsigned key)
 {
-    const unsigned x = (unsigned)key ^ ((unsigned)(key >> 32) * 0x85EBCA6Bu);
-    return (unsigned)(((unsigned long long)(x * 0x9E3779B1u) * SLOTS) >> 32);
+    return (unsigned)(((unsigned long long)(key * 0x9E3779B1u) * SLOTS) >> 32);
 }
 
 template <int SLOTS>
@@ -313,16 +359,19 @@ void batch_kernel(BatchArgs a)
     constexpr int kLP = (kMaxEntries + kBT - 1) / kBT;   // list entries per thread
     constexpr int kWords = Acc<MODE>::words;
     extern __shared__ unsigned long long lds[];
-    unsigned long long *tkey = lds;
-    unsigned long long *tacc0 = tkey + SLOTS;
+    unsigned long long *tacc0 = lds;
     unsigned long long *tacc1 = tacc0 + SLOTS;  // used when kWords == 2
+    unsigned int *tkey = reinterpret_cast<unsigned int *>(tacc0 + SLOTS * kWords);
     // slots in use, in insertion order (s_occ entries)
-    unsigned short *tlist = reinterpret_cast<unsigned short *>(tacc0 + SLOTS * kWords);
+    unsigned short *tlist = reinterpret_cast<unsigned short *>(tkey + SLOTS);
     unsigned int *lhist = reinterpret_cast<unsigned int *>(tlist + SLOTS);
 
     __shared__ long long s_seg;
     __shared__ int s_occ;
     __shared__ int s_range[2];
+    // the segment's shift (nm), read only by the rare exact path and at
+    // flushes, so that it occupies no registers through the point loop
+    __shared__ double s_shift[3];
     // entries each wave listed since the last flush, published at each
     // occupancy check into alternating halves
     __shared__ int s_wcnt[2][kBT / 64];
@@ -337,14 +386,15 @@ void batch_kernel(BatchArgs a)
     // per wave (uniform): entries it listed from new points, entries retired
     unsigned long long n_created = 0, n_retired = 0;
     for (int s = tid; s < SLOTS; s += kBT) {
-        tkey[s] = kEmpty;
+        tkey[s] = kEmpty32;
         tacc0[s] = 0;
         if (kWords == 2) tacc1[s] = 0;
     }
     for (int w = tid; w < a.hist_lds_words * a.hist_copies; w += kBT) lhist[w] = 0;
     // this wave's copy of the privatised histograms (more copies, fewer
     // waves contending for one hot bin's LDS word)
-    unsigned int *const lhist_w = lhist + ((tid >> 6) % a.hist_copies) * a.hist_lds_words;
+    // (the wave index is uniform: the offset stays in a scalar register)
+    unsigned int *const lhist_w = lhist + uniform_i32(((tid >> 6) % a.hist_copies) * a.hist_lds_words);
     if (tid == 0) s_occ = 0;
     __syncthreads();
     // upper bound on the counts added to any LDS bin since the last spill
@@ -353,6 +403,8 @@ void batch_kernel(BatchArgs a)
     // the next check uses (both uniform)
     int my_cnt = 0;
     int par = 0;
+    // the current slab's key box (uniform)
+    SlabBox box{0u, 0u, 0u, 1u, 1u, 1u, 1u};
 
     // Appends the slots that lanes newly occupied (slot >= 0) to the list:
     // one LDS atomic per wave reserves the positions.
@@ -381,14 +433,14 @@ void batch_kernel(BatchArgs a)
         }
     };
 
-    // Flushes entries with key < bound (all entries when bound == kEmpty):
+    // Flushes entries with key < bound (all entries when bound == kEmpty32):
     // complete sums are converted to lineal energy and binned; the rest are
     // re-inserted into the cleared table.  Threads walk the dense list of
     // used slots, so the work follows the entries, not the table size, and
     // the bin edges of two entries are loaded before either is compared.
     // after_check: called right after an occupancy check's barrier, which
     // already ordered every wave's appends before this read of s_occ.
-    auto flush = [&](unsigned long long bound, bool after_check) {
+    auto flush = [&](unsigned bound, bool after_check) {
         // raised issue priority while the workgroup's waves are tied together
         // by the flush barriers (measured: about 1 % on cfg 4)
         __builtin_amdgcn_s_setprio(2);
@@ -398,9 +450,10 @@ void batch_kernel(BatchArgs a)
             if (tid == 0) atomicOr(&a.ctl->error, 2);
             n = kLP * kBT < SLOTS ? kLP * kBT : SLOTS;
         }
-        unsigned long long kk[kLP], k0[kLP], k1[kLP];
+        unsigned kk[kLP];
+        unsigned long long k0[kLP], k1[kLP];
 #pragma unroll
-        for (int r = 0; r < kLP; ++r) kk[r] = kEmpty;
+        for (int r = 0; r < kLP; ++r) kk[r] = kEmpty32;
 #pragma unroll
         for (int r0 = 0; r0 < kLP; r0 += 2) {
             // rows past the list: nothing to read, convert or bin (n is uniform)
@@ -418,7 +471,7 @@ void batch_kernel(BatchArgs a)
                     const int sl = min((int)tlist[idx], SLOTS - 1);
                     // read and clear in one exchange each (the slot is this
                     // thread's alone until the barrier)
-                    const unsigned long long k = atomicExch(&tkey[sl], kEmpty);
+                    const unsigned k = atomicExch(&tkey[sl], kEmpty32);
                     const unsigned long long v0 = atomicExch(&tacc0[sl], 0ull);
                     const unsigned long long v1 = (kWords == 2) ? atomicExch(&tacc1[sl], 0ull) : 0ull;
                     em[u] = k < bound;
@@ -434,6 +487,9 @@ void batch_kernel(BatchArgs a)
                             const long long tk = s_seg;
                             a.rec_seg[p] = (int64_t)a.order[tk / a.n_os] * a.n_os + tk % a.n_os;
                             a.rec_key[p] = k;
+                            a.rec_box[3 * p] = box.kx0 | ((unsigned long long)box.ky0 << 32);
+                            a.rec_box[3 * p + 1] = box.kz0 | ((unsigned long long)box.nx << 32);
+                            a.rec_box[3 * p + 2] = box.ny;
                             a.rec_sum[p] = vv[u];
                         }
                     }
@@ -477,12 +533,12 @@ void batch_kernel(BatchArgs a)
 #pragma unroll
         for (int r = 0; r < kLP; ++r) {
             placed[r] = -1;
-            if (kk[r] != kEmpty) {
+            if (kk[r] != kEmpty32) {
                 unsigned int h = slot_hash<SLOTS>(kk[r]);
 #pragma unroll 1
                 for (int probe = 0; probe < SLOTS; ++probe) {
-                    const unsigned long long prev = atomicCAS(&tkey[h], kEmpty, kk[r]);
-                    if (prev == kEmpty) break;
+                    const unsigned prev = atomicCAS(&tkey[h], kEmpty32, kk[r]);
+                    if (prev == kEmpty32) break;
                     h = next_slot<SLOTS>(h);
                 }
                 placed[r] = (int)h;
@@ -516,175 +572,254 @@ void batch_kernel(BatchArgs a)
         if (seg >= a.n_seg) break;
         const int32_t t = a.order[seg / a.n_os];
         const int32_t os = (int32_t)(seg % a.n_os);
-        const TrackDesc td = a.tracks[t];
-        const GeomDev g = a.geoms[td.geom];
+        const TrackDesc td = load_uniform(a.tracks + t);
+        const GeomDev g = load_uniform(a.geoms + td.geom);
         const Shift sh = shift_for(a.seed, a.track_index_base + (uint32_t)t, (uint32_t)os, a.shift_xy,
                                    a.shift_z, g.L, g.H);
-        // the shift in lattice units (ST_F32 fast path)
-        const double sqx = sh.x * g.invD, sqy = sh.y * g.invD, sqz = sh.z * g.invD;
         // x/y bounds cull: fl(x + s) is monotone in x
         if (!((td.xmax + sh.x) > -g.H && (td.xmin + sh.x) < g.H && (td.ymax + sh.y) > -g.H &&
               (td.ymin + sh.y) < g.H))
             continue;
-        // in-voxel block range: blocks whose z span can reach inside (-H, H)
-        if (tid < 64) {
-            const double *zf = a.zfirst + td.block0;
-            const int nb = td.nblocks;
-            auto zs_of = [&](double z) { return (z + g.H) + sh.z; };
-            const int b_lo = wave_lower_bound(nb, [&](int b) {
-                const double zhi = (b + 1 < nb) ? zf[b + 1] : td.zmax;
-                return zs_of(zhi) > -g.H;
-            });
-            const int b_hi = wave_lower_bound(nb, [&](int b) { return !(zs_of(zf[b]) < g.H); });
-            if (tid == 0) {
-                s_range[0] = b_lo;
-                s_range[1] = b_hi;
-            }
+        // The segment's box of sphere indices.  fl(fl(c + s) - gso) / D is
+        // monotone in the coordinate c, so rint of it lies between floor and
+        // ceil of the bounds' quotients, and the exact path's nearest_index
+        // (which may round a half-way quotient the other way) within one more.
+        const unsigned N = (unsigned)g.N;
+        const double fN1 = (double)(N - 1u);
+        const double zmin = load_uniform(a.zfirst + td.block0);
+        const double bx0 = fmax(0.0, floor(((td.xmin + sh.x) - g.gso) * g.invD) - 1.0);
+        const double bx1 = fmin(fN1, ceil(((td.xmax + sh.x) - g.gso) * g.invD) + 1.0);
+        const double by0 = fmax(0.0, floor(((td.ymin + sh.y) - g.gso) * g.invD) - 1.0);
+        const double by1 = fmin(fN1, ceil(((td.ymax + sh.y) - g.gso) * g.invD) + 1.0);
+        const double bz0 = fmax(0.0, floor((((zmin + g.H) + sh.z) - g.gso) * g.invD) - 1.0);
+        const double bz1 = fmin(fN1, ceil((((td.zmax + g.H) + sh.z) - g.gso) * g.invD) + 1.0);
+        if (!(bx0 <= bx1 && by0 <= by1 && bz0 <= bz1)) continue;
+        // (uniform values computed by the vector unit: moved to scalar registers)
+        const unsigned kx0 = uniform_u32((unsigned)bx0), ky0 = uniform_u32((unsigned)by0);
+        const unsigned kz_lo = uniform_u32((unsigned)bz0), kz_hi = uniform_u32((unsigned)bz1);
+        const unsigned nx = uniform_u32((unsigned)bx1) - kx0 + 1u, ny = uniform_u32((unsigned)by1) - ky0 + 1u;
+        const unsigned nz_all = kz_hi - kz_lo + 1u;
+        const unsigned long long nxy = (unsigned long long)nx * ny;
+        if (nxy > kMaxKeys) {   // excluded by the library layout; reported, never scored wrong
+            if (tid == 0) atomicOr(&a.ctl->error, 4);
+            continue;
         }
-        __syncthreads();
-        __builtin_amdgcn_s_setprio(0);
-        const int b_lo = uniform_i32(s_range[0]), b_hi = uniform_i32(s_range[1]);
-        if (b_lo >= b_hi) continue;
-        const int64_t p_end = td.offset + min((int64_t)b_hi * kBlk, (int64_t)td.n);
-        const int64_t p_safe = td.offset + (int64_t)b_lo * kBlk;   // a point of the range
-        binned += (long long)(b_hi - b_lo) * kBlk;
-        // Software pipeline: the points of the next iteration are loaded
-        // before the current ones are scored.  Lanes past the end of the
-        // range load a valid point and are masked off.
-        using PointT = typename std::conditional<MODE == ST_F64, double4, float4>::type;
-        PointT nxt[kPPT];
-        auto fetch = [&](int b, PointT *dst) {
-#pragma unroll
-            for (int j = 0; j < kPPT; ++j) {
-                int64_t i = td.offset + (int64_t)(b + j) * kBlk + tid;
-                if (!(b + j < b_hi && i < p_end)) i = p_safe;
-                if constexpr (MODE == ST_F64) dst[j] = make_double4(a.x64[i], a.y64[i], a.z64[i], a.e64[i]);
-                else dst[j] = a.p32[i];
+        // layers per slab: the whole segment unless its box exceeds kMaxKeys keys
+        const unsigned layers = (nxy * nz_all <= kMaxKeys) ? nz_all : (unsigned)(kMaxKeys / nxy);
+        const bool multi = layers < nz_all;
+        // the shift in lattice units
+        const double sqx = sh.x * g.invD, sqy = sh.y * g.invD, sqz = sh.z * g.invD;
+        // Exact membership of one point and its key in the current slab's
+        // box.  An index equal to N (a point within rounding of the voxel's
+        // upper face) is carried into the next index, giving the indices of
+        // the sphere whose key the reference's sum kx + N ky + N^2 kz names;
+        // both then accumulate into that one entry, as in the reference.
+        auto exact_local = [&](double x, double y, double zp, unsigned &key) -> bool {
+            unsigned gx, gy, gz;
+            if (!exact_member(x, y, zp, g, s_shift[0], s_shift[1], s_shift[2], gx, gy, gz)) return false;
+            const bool alias = max(max(gx, gy), gz) >= N;
+            if (gx >= N) {
+                gx -= N;
+                ++gy;
             }
+            if (gy >= N) {
+                gy -= N;
+                ++gz;
+            }
+            const unsigned lx = gx - box.kx0, ly = gy - box.ky0, lz = gz - box.kz0;
+            // every in-sphere point lies in the segment's box by construction;
+            // only a carried index can leave it (or, across slabs, move to
+            // another slab): reported, never scored wrong
+            if (lx >= box.nx || ly >= box.ny || gz - kz_lo >= nz_all || (alias && multi))
+                atomicOr(&a.ctl->error, 4);
+            if (!(lx < box.nx && ly < box.ny && lz < box.nz)) return false;
+            key = (lz * box.ny + ly) * box.nx + lx;
+            return true;
         };
-        fetch(b_lo, nxt);
-        int since = 0;
-        for (int b = b_lo; b < b_hi; b += kPPT) {
-            PointT cur[kPPT];
-#pragma unroll
-            for (int j = 0; j < kPPT; ++j) cur[j] = nxt[j];
-            if (b + kPPT < b_hi) fetch(b + kPPT, nxt);
-            const int64_t c = td.offset + (int64_t)b * kBlk;
-            int fresh[kPPT];
-#pragma unroll
-            for (int j = 0; j < kPPT; ++j) fresh[j] = -1;
-#pragma unroll
-            for (int j = 0; j < kPPT; ++j) {
-                if (b + j >= b_hi) break;
-                const int64_t i = c + (int64_t)j * kBlk + tid;
-                const bool valid = i < p_end;
-                bool in = false;
-                unsigned long long key = 0;
-                long long v0 = 0, v1 = 0;
-                if constexpr (MODE == ST_F64) {
-                    // lattice units straight from the FP64 data; decisions
-                    // within the rounding bound are redone exactly
-                    const double qx = fma(cur[j].x, g.invD, sqx - g.qc);
-                    const double qy = fma(cur[j].y, g.invD, sqy - g.qc);
-                    const double qz = fma(cur[j].z, g.invD, (sqz + g.H * g.invD) - g.qc);
-                    const double tx = qx + kRoundMagic, ty = qy + kRoundMagic, tz = qz + kRoundMagic;
-                    const double ux = (tx - kRoundMagic) - qx, uy = (ty - kRoundMagic) - qy,
-                                 uz = (tz - kRoundMagic) - qz;
-                    const double distq = fma(ux, ux, fma(uy, uy, uz * uz));
-                    // low words of t: the indices (|q| < 2^30, checked per track)
-                    const unsigned kx = (unsigned)__double2loint(tx), ky = (unsigned)__double2loint(ty),
-                                   kz = (unsigned)__double2loint(tz);
-                    const bool box = max(max(kx, ky), kz) < (unsigned)g.N;
-                    const bool near = (fmax(fmax(fabs(ux), fabs(uy)), fabs(uz)) >= g.fm) |
-                                      (fabs(distq - g.r2q) <= g.mrgq);
-                    in = valid & box & !near & (distq < g.r2q);
-                    key = lattice_key(kx, ky, kz, (unsigned)g.N);
-                    if (valid & near) {
-                        // redo exactly unless clearly outside (an index beyond [-1, N])
-                        in = max(max(kx + 1u, ky + 1u), kz + 1u) <= (unsigned)g.N + 1u &&
-                             exact_member(cur[j].x, cur[j].y, cur[j].z + g.H, g, sh.x, sh.y, sh.z, key);
-                    }
-                    if (in) Acc<ST_F64>::split(cur[j].w / g.denom, v0, v1);
-                } else {
-                    // block-uniform terms (scalar loads)
-                    const BlockF32 bk = load_block(a.blocks, td.block0 + b + j);
-                    const double cqx = bk.oqx + sqx, cqy = bk.oqy + sqy, cqz = bk.oqz + sqz;
-                    // per point, in lattice units: |q - reference a / D| <= dl / D
-                    const double qx = fma((double)cur[j].x, g.invD, cqx);
-                    const double qy = fma((double)cur[j].y, g.invD, cqy);
-                    const double qz = fma((double)cur[j].z, g.invD, cqz);
-                    const double tx = qx + kRoundMagic, ty = qy + kRoundMagic, tz = qz + kRoundMagic;
-                    const double ux = (tx - kRoundMagic) - qx, uy = (ty - kRoundMagic) - qy,
-                                 uz = (tz - kRoundMagic) - qz;
-                    const double distq = fma(ux, ux, fma(uy, uy, uz * uz));
-                    // low words of t: the indices (|q| < 2^30, checked per track)
-                    const unsigned kx = (unsigned)__double2loint(tx), ky = (unsigned)__double2loint(ty),
-                                   kz = (unsigned)__double2loint(tz);
-                    const bool box = max(max(kx, ky), kz) < (unsigned)g.N;
-                    const bool near = (fmax(fmax(fabs(ux), fabs(uy)), fabs(uz)) >= bk.fm) |
-                                      (fabs(distq - g.r2q) <= bk.mrgq);
-                    in = valid & box & !near & (distq < g.r2q);
-                    key = lattice_key(kx, ky, kz, (unsigned)g.N);
-                    if (valid & near) {
-                        // any decision within the error bound is redone exactly
-                        // from FP64, unless clearly outside (an index beyond [-1, N])
-                        const bool cand = max(max(kx + 1u, ky + 1u), kz + 1u) <= (unsigned)g.N + 1u;
-                        n_flagged += cand;
-                        in = cand && exact_member(a.x64[i], a.y64[i], a.z64[i] + g.H, g, sh.x, sh.y, sh.z, key);
-                    }
-                    v0 = rint_to_i64((double)cur[j].w * g.yscale);
-                }
-                n_scored += valid;
-                n_in += in;
-                if (in) {
-                    // linear probing; the loop is kept rolled: nearly every
-                    // probe ends at its first slot, and an unrolled loop pays
-                    // exec-mask merges for every possible exit
-                    unsigned int h = slot_hash<SLOTS>(key);
-                    unsigned long long prev = atomicCAS(&tkey[h], kEmpty, key);
-#pragma unroll 1
-                    for (int probe = 1; probe < SLOTS && prev != kEmpty && prev != key; ++probe) {
-                        h = next_slot<SLOTS>(h);
-                        prev = atomicCAS(&tkey[h], kEmpty, key);
-                    }
-                    if (prev == kEmpty || prev == key) {
-                        if (prev == kEmpty) fresh[j] = (int)h;
-                        atomicAdd(&tacc0[h], (unsigned long long)v0);
-                        if (kWords == 2) atomicAdd(&tacc1[h], (unsigned long long)v1);
-                        bal_in += Acc<MODE>::balance((unsigned long long)v0, (unsigned long long)v1);
-                    } else {
-                        atomicOr(&a.ctl->error, 1);
-                    }
-                }
-            }
-            append(fresh, std::integral_constant<int, kPPT>(), true);
-            const int64_t next = c + (int64_t)kPPT * kBlk;
-            if (++since < kCheck || next >= p_end) continue;
-            since = 0;
-            // Every wave must take the same decision.  Each publishes its own
-            // count into this check's half of s_wcnt and sums all of them
-            // after one barrier: a wave that runs on and appends more writes
-            // only its count for the next check, into the other half, so the
-            // counts read here cannot change under a slower wave (which would
-            // otherwise flush alone).  The sum equals s_occ at the barrier.
-            if (lane == 0) s_wcnt[par][tid >> 6] = my_cnt;
-            __syncthreads();
-            int occ = 0;
-#pragma unroll
-            for (int w = 0; w < kBT / 64; ++w) occ += s_wcnt[par][w];
-            par ^= 1;
-            if (occ >= kFlushAt) {
-                // every later point lies in a layer >= the next point's exact
-                // layer index (z order), and a key's layer (key / N^2) is never
-                // below its point's layer: entries with smaller keys are complete
-                const double zs = (a.z64[next] + g.H) + sh.z;
-                double lb = nearest_index(zs - g.gso, g.D, g.invD);
-                if (!(lb > 0.0)) lb = 0.0;
-                const unsigned long long bound = (unsigned long long)lb * g.N2;
-                flush(bound, true);
-            }
+        // (read after the block-range barrier below; rewritten only after
+        // the next segment's ticket barriers)
+        if (tid == 0) {
+            s_shift[0] = sh.x;
+            s_shift[1] = sh.y;
+            s_shift[2] = sh.z;
         }
-        flush(kEmpty, false);
+        for (unsigned k0 = kz_lo;;) {
+            const unsigned k1 = min(kz_hi, k0 + layers - 1u);
+            box = SlabBox{kx0, ky0, k0, nx, ny, k1 - k0 + 1u, (unsigned)nxy};
+            // lattice-unit offsets that make the rounded coordinates local indices
+            const double ox = (MODE == ST_F64 ? sqx - g.qc : sqx) - (double)kx0;
+            const double oy = (MODE == ST_F64 ? sqy - g.qc : sqy) - (double)ky0;
+            const double oz = (MODE == ST_F64 ? (sqz + g.H * g.invD) - g.qc : sqz) - (double)k0;
+            // a point of layer k in [k0, k1] has zs - gso within half a diameter
+            // of k D: the slab's points lie strictly inside (zlo, zhi)
+            const double zlo = fmax(-g.H, g.gso + ((double)k0 - 1.0) * g.D);
+            const double zhi = fmin(g.H, g.gso + ((double)k1 + 1.0) * g.D);
+            // block range: blocks whose z span can reach inside (zlo, zhi)
+            __builtin_amdgcn_s_setprio(2);
+            if (tid < 64) {
+                const double *zf = a.zfirst + td.block0;
+                const int nb = td.nblocks;
+                auto zs_of = [&](double z) { return (z + g.H) + sh.z; };
+                const int r_lo = wave_lower_bound(nb, [&](int b) {
+                    const double zhb = (b + 1 < nb) ? zf[b + 1] : td.zmax;
+                    return zs_of(zhb) > zlo;
+                });
+                const int r_hi = wave_lower_bound(nb, [&](int b) { return !(zs_of(zf[b]) < zhi); });
+                if (tid == 0) {
+                    s_range[0] = r_lo;
+                    s_range[1] = r_hi;
+                }
+            }
+            __syncthreads();
+            __builtin_amdgcn_s_setprio(0);
+            const int b_lo = uniform_i32(s_range[0]), b_hi = uniform_i32(s_range[1]);
+            // the next slab's set-up writes s_range after every wave's read
+            if (multi) __syncthreads();
+            if (b_lo < b_hi) {
+                const int64_t p_end = td.offset + min((int64_t)b_hi * kBlk, (int64_t)td.n);
+                const int64_t p_safe = td.offset + (int64_t)b_lo * kBlk;   // a point of the range
+                binned += (long long)(b_hi - b_lo) * kBlk;
+                // Software pipeline: the points of the next iteration are loaded
+                // before the current ones are scored.  Lanes past the end of the
+                // range load a valid point and are masked off.
+                using PointT = typename std::conditional<MODE == ST_F64, double4, float4>::type;
+                PointT nxt[kPPT];
+                auto fetch = [&](int b, PointT *dst) {
+#pragma unroll
+                    for (int j = 0; j < kPPT; ++j) {
+                        int64_t i = td.offset + (int64_t)(b + j) * kBlk + tid;
+                        if (!(b + j < b_hi && i < p_end)) i = p_safe;
+                        if constexpr (MODE == ST_F64) dst[j] = make_double4(a.x64[i], a.y64[i], a.z64[i], a.e64[i]);
+                        else dst[j] = a.p32[i];
+                    }
+                };
+                fetch(b_lo, nxt);
+                int since = 0;
+                for (int b = b_lo; b < b_hi; b += kPPT) {
+                    PointT cur[kPPT];
+#pragma unroll
+                    for (int j = 0; j < kPPT; ++j) cur[j] = nxt[j];
+                    if (b + kPPT < b_hi) fetch(b + kPPT, nxt);
+                    const int64_t c = td.offset + (int64_t)b * kBlk;
+                    int fresh[kPPT];
+#pragma unroll
+                    for (int j = 0; j < kPPT; ++j) fresh[j] = -1;
+#pragma unroll
+                    for (int j = 0; j < kPPT; ++j) {
+                        if (b + j >= b_hi) break;
+                        const int64_t i = c + (int64_t)j * kBlk + tid;
+                        const bool valid = i < p_end;
+                        bool in = false;
+                        unsigned key = 0;
+                        long long v0 = 0, v1 = 0;
+                        // lattice units relative to the slab's box corner
+                        double qx, qy, qz, fm, mrgq;
+                        if constexpr (MODE == ST_F64) {
+                            // straight from the FP64 data; decisions within the
+                            // rounding bound are redone exactly
+                            qx = fma(cur[j].x, g.invD, ox);
+                            qy = fma(cur[j].y, g.invD, oy);
+                            qz = fma(cur[j].z, g.invD, oz);
+                            fm = g.fm;
+                            mrgq = g.mrgq;
+                        } else {
+                            // block-uniform terms (scalar loads); per point
+                            // |q - reference quotient| <= dl / D
+                            const BlockF32 bk = load_block(a.blocks, td.block0 + b + j);
+                            qx = fma((double)cur[j].x, g.invD, bk.oqx + ox);
+                            qy = fma((double)cur[j].y, g.invD, bk.oqy + oy);
+                            qz = fma((double)cur[j].z, g.invD, bk.oqz + oz);
+                            fm = bk.fm;
+                            mrgq = bk.mrgq;
+                        }
+                        const double tx = qx + kRoundMagic, ty = qy + kRoundMagic, tz = qz + kRoundMagic;
+                        const double ux = (tx - kRoundMagic) - qx, uy = (ty - kRoundMagic) - qy,
+                                     uz = (tz - kRoundMagic) - qz;
+                        const double distq = fma(ux, ux, fma(uy, uy, uz * uz));
+                        // low words of t: the local indices (|q| < 2^30, checked per track)
+                        const unsigned lx = (unsigned)__double2loint(tx), ly = (unsigned)__double2loint(ty),
+                                       lz = (unsigned)__double2loint(tz);
+                        // inside the slab's box, which lies inside the voxel
+                        const bool inbox = (lx < box.nx) & (ly < box.ny) & (lz < box.nz);
+                        const bool near =
+                            (fmax(fmax(fabs(ux), fabs(uy)), fabs(uz)) >= fm) | (fabs(distq - g.r2q) <= mrgq);
+                        in = valid & inbox & !near & (distq < g.r2q);
+                        key = (lz * box.ny + ly) * box.nx + lx;
+                        if (valid & near) {
+                            // any decision within the error bound is redone exactly
+                            // from FP64, unless clearly outside (an index beyond the
+                            // box by more than one)
+                            const bool cand =
+                                (lx + 1u <= box.nx + 1u) & (ly + 1u <= box.ny + 1u) & (lz + 1u <= box.nz + 1u);
+                            // (statistic of the FP32 path: decisions sent to FP64)
+                            if constexpr (MODE == ST_F32) n_flagged += cand;
+                            if constexpr (MODE == ST_F64)
+                                in = cand && exact_local(cur[j].x, cur[j].y, cur[j].z + g.H, key);
+                            else
+                                in = cand && exact_local(a.x64[i], a.y64[i], a.z64[i] + g.H, key);
+                        }
+                        if constexpr (MODE == ST_F64) {
+                            if (in) Acc<ST_F64>::split(cur[j].w / g.denom, v0, v1);
+                        } else {
+                            v0 = rint_to_i64((double)cur[j].w * g.yscale);
+                        }
+                        n_scored += valid;
+                        n_in += in;
+                        if (in) {
+                            // linear probing; the loop is kept rolled: nearly every
+                            // probe ends at its first slot, and an unrolled loop pays
+                            // exec-mask merges for every possible exit
+                            unsigned int h = slot_hash<SLOTS>(key);
+                            unsigned prev = atomicCAS(&tkey[h], kEmpty32, key);
+#pragma unroll 1
+                            for (int probe = 1; probe < SLOTS && prev != kEmpty32 && prev != key; ++probe) {
+                                h = next_slot<SLOTS>(h);
+                                prev = atomicCAS(&tkey[h], kEmpty32, key);
+                            }
+                            if (prev == kEmpty32 || prev == key) {
+                                if (prev == kEmpty32) fresh[j] = (int)h;
+                                atomicAdd(&tacc0[h], (unsigned long long)v0);
+                                if (kWords == 2) atomicAdd(&tacc1[h], (unsigned long long)v1);
+                                bal_in += Acc<MODE>::balance((unsigned long long)v0, (unsigned long long)v1);
+                            } else {
+                                atomicOr(&a.ctl->error, 1);
+                            }
+                        }
+                    }
+                    append(fresh, std::integral_constant<int, kPPT>(), true);
+                    const int64_t next = c + (int64_t)kPPT * kBlk;
+                    if (++since < kCheck || next >= p_end) continue;
+                    since = 0;
+                    // Every wave must take the same decision.  Each publishes its own
+                    // count into this check's half of s_wcnt and sums all of them
+                    // after one barrier: a wave that runs on and appends more writes
+                    // only its count for the next check, into the other half, so the
+                    // counts read here cannot change under a slower wave (which would
+                    // otherwise flush alone).  The sum equals s_occ at the barrier.
+                    if (lane == 0) s_wcnt[par][tid >> 6] = my_cnt;
+                    __syncthreads();
+                    int occ = 0;
+#pragma unroll
+                    for (int w = 0; w < kBT / 64; ++w) occ += s_wcnt[par][w];
+                    par ^= 1;
+                    if (occ >= kFlushAt) {
+                        // every later point lies in a layer >= the next point's exact
+                        // layer index (z order), and a key's layer is never below its
+                        // point's layer: entries in the slab's layers below it are complete
+                        // (the next point lies in the range: |its layer| < 2^30)
+                        const double zs = (a.z64[next] + g.H) + s_shift[2];
+                        const int lb = uniform_i32((int)nearest_index(zs - g.gso, g.D, g.invD)) - (int)box.kz0;
+                        unsigned bound = 0u;
+                        if (lb >= (int)box.nz) bound = kEmpty32;
+                        else if (lb > 0) bound = (unsigned)lb * box.nxy;
+                        flush(bound, true);
+                    }
+                }
+                flush(kEmpty32, false);
+            }
+            if (k1 >= kz_hi) break;
+            k0 = k1 + 1u;
+        }
         if (a.hist_lds_words && binned > 0x40000000ll) {
             // keep the 32-bit LDS counters far from overflow
 #pragma unroll
@@ -770,7 +905,8 @@ const void *pick_kernel(int mode, int slots, int ppt)
 size_t lds_bytes(int mode, int slots, int hist_words)
 {
     const int words = (mode == ST_F64) ? 2 : 1;
-    return (size_t)slots * (8 * (1 + words) + 2) + (size_t)hist_words * 4;
+    // per slot: the sums, the 32-bit key, the list entry
+    return (size_t)slots * (8 * words + 4 + 2) + (size_t)hist_words * 4;
 }
 
 template <typename T>
@@ -858,6 +994,7 @@ struct st_batch {
     int64_t rec_cap = 0;
     int64_t *rec_seg = nullptr;
     unsigned long long *rec_key = nullptr;
+    unsigned long long *rec_box = nullptr;
     double *rec_sum = nullptr;
     bool graph_ready = false;
     hipGraph_t graph = nullptr;
@@ -927,6 +1064,7 @@ int launch_fast(st_batch *b)
     a.rec_cap = b->rec_cap;
     a.rec_seg = b->rec_seg;
     a.rec_key = b->rec_key;
+    a.rec_box = b->rec_box;
     a.rec_sum = b->rec_sum;
     return launch_pass(b->lib->precision, a, b->ln, b->stream);
 }
@@ -1151,7 +1289,10 @@ int read_ctl(const BatchCtl *ctl, hipStream_t stream, BatchCtl *out)
 int check_ctl(const BatchCtl &c)
 {
     if (c.error)
-        return set_error(ST_ESTATE, "batched kernel reported a full sphere table or entry list (code %d)", c.error);
+        return set_error(ST_ESTATE,
+                         "batched kernel reported an error (code %d: 1 full sphere table, 2 entry list overrun, "
+                         "4 key outside its segment's box)",
+                         c.error);
     if (c.stats[3] != c.stats[4] || c.stats[5] != c.stats[6])
         return set_error(ST_ESTATE,
                          "batched kernel lost sphere-table entries: energy words in %016llx / out %016llx, "
@@ -1554,14 +1695,17 @@ int st_batch_record(st_batch *b, int64_t capacity)
     if (!b || capacity < 0) return set_error(ST_EINVAL, "st_batch_record: bad arguments");
     (void)hipFree(b->rec_seg);
     (void)hipFree(b->rec_key);
+    (void)hipFree(b->rec_box);
     (void)hipFree(b->rec_sum);
     b->rec_seg = nullptr;
     b->rec_key = nullptr;
+    b->rec_box = nullptr;
     b->rec_sum = nullptr;
     b->rec_cap = 0;
     if (capacity == 0) return ST_OK;
     ST_ALLOC_TRY(dev_malloc((void **)&b->rec_seg, capacity * sizeof(int64_t)));
     ST_ALLOC_TRY(dev_malloc((void **)&b->rec_key, capacity * sizeof(unsigned long long)));
+    ST_ALLOC_TRY(dev_malloc((void **)&b->rec_box, 3 * capacity * sizeof(unsigned long long)));
     ST_ALLOC_TRY(dev_malloc((void **)&b->rec_sum, capacity * sizeof(double)));
     b->rec_cap = capacity;
     return ST_OK;
@@ -1578,7 +1722,24 @@ int st_batch_record_fetch(st_batch *b, int64_t *segments, uint64_t *keys, double
     if ((int64_t)c > b->rec_cap) return set_error(ST_ERANGE, "st_batch_record_fetch: %llu records, capacity %lld",
                                                  c, (long long)b->rec_cap);
     if (c && segments) ST_HIP_TRY(hipMemcpy(segments, b->rec_seg, c * sizeof(int64_t), hipMemcpyDeviceToHost));
-    if (c && keys) ST_HIP_TRY(hipMemcpy(keys, b->rec_key, c * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    if (c && keys) {
+        // the reference's key from the local key and its slab's box
+        std::vector<int64_t> seg(c);
+        std::vector<unsigned long long> loc(c), bx(3 * c);
+        ST_HIP_TRY(hipMemcpy(seg.data(), b->rec_seg, c * sizeof(int64_t), hipMemcpyDeviceToHost));
+        ST_HIP_TRY(hipMemcpy(loc.data(), b->rec_key, c * sizeof(uint64_t), hipMemcpyDeviceToHost));
+        ST_HIP_TRY(hipMemcpy(bx.data(), b->rec_box, 3 * c * sizeof(uint64_t), hipMemcpyDeviceToHost));
+        const int64_t n_os = std::max(b->opts.n_oversamples, 1);
+        for (unsigned long long i = 0; i < c; ++i) {
+            const TrackDesc &td = b->lib->tracks[seg[i] / n_os];
+            const unsigned long long N = (unsigned long long)b->lib->geoms[td.geom].n_linear;
+            const unsigned long long kx0 = bx[3 * i] & 0xFFFFFFFFull, ky0 = bx[3 * i] >> 32;
+            const unsigned long long kz0 = bx[3 * i + 1] & 0xFFFFFFFFull, nx = bx[3 * i + 1] >> 32;
+            const unsigned long long ny = bx[3 * i + 2];
+            const unsigned long long lx = loc[i] % nx, ly = (loc[i] / nx) % ny, lz = loc[i] / (nx * ny);
+            keys[i] = (kx0 + lx) + (ky0 + ly) * N + (kz0 + lz) * N * N;
+        }
+    }
     if (c && sums) ST_HIP_TRY(hipMemcpy(sums, b->rec_sum, c * sizeof(double), hipMemcpyDeviceToHost));
     return ST_OK;
 }
@@ -1594,7 +1755,7 @@ void st_batch_destroy(st_batch *b)
         if (e) (void)hipEventDestroy(e);
     st_track_free(&b->scratch);
     st_pairs_free(&b->pairs);
-    void *ptrs[] = {b->ctl, b->rec_seg, b->rec_key, b->rec_sum};
+    void *ptrs[] = {b->ctl, b->rec_seg, b->rec_key, b->rec_box, b->rec_sum};
     for (void *p : ptrs) (void)hipFree(p);
     delete b;
 }

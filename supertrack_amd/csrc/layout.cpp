@@ -123,6 +123,12 @@ void fill_chunk(const double *x, const double *y, const double *z, const double 
                                           std::fabs(out.z[o] + g.half_length),
                                           std::fabs(out.z[o + n - 1] + g.half_length)});
             fast = (cmax + g.half_length + std::fabs(g.gso)) / g.diameter < 1073741824.0;
+            // one layer of the segment's key box (x-cells times y-cells, for
+            // any shift) must fit the 32-bit keys of the sphere table
+            const double nl = (double)g.n_linear;
+            const double cx = std::min(nl, std::floor((xmax - xmin) / g.diameter) + 6.0);
+            const double cy = std::min(nl, std::floor((ymax - ymin) / g.diameter) + 6.0);
+            fast = fast && cx * cy <= 4294967294.0;
         }
         if (fast) {
             // entries kept by a flush come from the next point's layer (or, for

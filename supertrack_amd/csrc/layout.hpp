@@ -74,7 +74,8 @@ int64_t plan_chunk(const int64_t *offsets, int64_t n_tracks, const int32_t *geom
 // Sorts each planned track's points by z (stable; NaN last) into out, and
 // completes its record: x/y bounds, last z, and the fast-path eligibility
 // (finite data, at most kMaxFastPoints points, fixed-point headroom, indices
-// below 2^30 for any shift, at most kMaxWindow spheres in any z-window two
+// below 2^30 for any shift, at most 2^32 - 2 spheres in one layer of the
+// track's x/y extent, at most kMaxWindow spheres in any z-window two
 // diameters wide).  ST_F32 also gets the FP32 points and block records.
 void fill_chunk(const double *x, const double *y, const double *z, const double *edep, const int64_t *offsets,
                 int64_t n_tracks, const st_geometry *geoms, int precision, int64_t pbase, int64_t bbase,

@@ -192,6 +192,34 @@ int main()
         std::printf("ok eligibility: NaN, infinite energy, dense plane, %lld points -> exact path; ordinary -> fast\n",
                     (long long)(kMaxFastPoints + 1));
     }
+    // 2b. eligibility: one layer of the track's x/y extent must hold at most
+    //     2^32 - 2 spheres (32-bit sphere keys); only possible for N > 65535
+    {
+        const std::vector<st_geometry> gw = {geometry(50000.0, 1.0)};   // N = 100000
+        Lib M;
+        auto two_points = [&](double span) {
+            M.x.push_back(-span / 2);
+            M.y.push_back(-span / 2);
+            M.z.push_back(0.0);
+            M.e.push_back(10.0);
+            M.x.push_back(span / 2);
+            M.y.push_back(span / 2);
+            M.z.push_back(1.0);
+            M.e.push_back(10.0);
+            M.off.push_back((int64_t)M.x.size());
+            M.geom.push_back(0);
+        };
+        two_points(65536.0);   // 65542^2 > 2^32 - 2: exact path
+        two_points(65520.0);   // 65526^2 < 2^32 - 2: fast
+        const int64_t nt = (int64_t)M.off.size() - 1;
+        std::vector<TrackDesc> tds(nt);
+        const int64_t nb = plan_chunk(M.off.data(), nt, M.geom.data(), 0, 0, tds.data());
+        ChunkLayout out;
+        fill_chunk(M.x.data(), M.y.data(), M.z.data(), M.e.data(), M.off.data(), nt, gw.data(), ST_F32, 0, 0, nb,
+                   tds.data(), out);
+        if (tds[0].flags != kFlagExact || tds[1].flags != 0u) fail("key-box eligibility", tds[0].flags, tds[1].flags);
+        std::printf("ok eligibility: a layer of 65542^2 spheres -> exact path; 65526^2 -> fast\n");
+    }
     // 3. the sliding-window bound against brute force
     for (int rep = 0; rep < 40; ++rep) {
         const int64_t n = 50 + (int64_t)(rng() % 600);

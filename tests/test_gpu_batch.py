@@ -580,3 +580,90 @@ def test_batch_histogram_copies_are_invariant():
     with pytest.raises(st.SuperTrackError, match="copies"):
         gpu_run(x, y, z, e, off, [(2.5e3, 10.0)], None, 2, 8, True, True, st.ST_F32,
                 [("lin", 3000, 0.01, 300.0), ("log", 1000, 1e-2, 1e3)], hist_copies=4)
+
+
+@pytest.mark.parametrize("precision", [st.ST_F64, st.ST_F32])
+@pytest.mark.parametrize("shift", [False, True])
+def test_batch_wide_tracks_scored_in_slabs(precision, shift):
+    """Tracks spread over the whole voxel of a 2000^3 lattice (1 nm spheres,
+    half-length 1 um): a segment's box of reachable spheres exceeds the 2^32 - 1
+    keys of the sphere table (about 2000^3 without shifts), so the kernel
+    scores it in slabs of layers.  Keys and sums equal the oracle's."""
+    rng = np.random.default_rng(11 + precision + 2 * shift)
+    H = 1.0e3
+    g = po.geometry(H, 1.0)
+    assert g.n_linear == 2000
+    n_tracks, per_track, n_os = 3, 20000, 3
+    x = rng.uniform(-H, H, n_tracks * per_track)
+    y = rng.uniform(-H, H, n_tracks * per_track)
+    z = rng.uniform(-2 * H, 0.0, n_tracks * per_track)   # z + H' inside the voxel without shifts
+    e = rng.uniform(1.0, 1000.0, n_tracks * per_track)
+    off = np.arange(n_tracks + 1, dtype=np.int64) * per_track
+    # the box of one unshifted segment: more spheres than 32-bit keys
+    assert (2 * H / g.diameter) ** 3 > 2**32
+    oracle = oracle_segments(x, y, z, e, off, [g], None, n_os, W.SEED, shift, shift)
+    n_rec = sum(len(k) for k, _ in oracle.values())
+    counts, rec, info, edges = gpu_run(x, y, z, e, off, [(H, 1.0)], None, n_os, W.SEED, shift, shift, precision,
+                                       [("log", 1000, 1e-2, 1e3)], record=n_rec + 1024)
+    assert info["exact_tracks"] == 0
+    n, _ = compare_records(rec, oracle, 1e-6 if precision == st.ST_F32 else 1e-12)
+    assert n > 10000
+    check_bins(counts, edges, oracle, 1e-6 if precision == st.ST_F32 else 1e-12)
+    n_pairs = sum(len(pk) for t in range(n_tracks)
+                  for pk, _, _, _ in po.track_segments(x[off[t]:off[t + 1]], y[off[t]:off[t + 1]],
+                                                       z[off[t]:off[t + 1]], e[off[t]:off[t + 1]], g, W.SEED, t,
+                                                       n_os, shift, shift))
+    assert info["points_in_sphere"] == n_pairs
+
+
+def _face_alias_track(partner):
+    """One point within an ulp of the +x voxel face, on the lattice line of
+    sphere row (ky, kz): its x index rounds to N, and the reference's key
+    kx + N ky + N^2 kz then names sphere (0, ky + 1, kz).  With `partner`, a
+    second point sits at the centre of that sphere (so the two share one sum
+    in the reference), plus an ordinary point in between."""
+    g = po.geometry(2.5e3, 10.0)
+    N, H, gso, D = g.n_linear, g.half_length, g.gso, g.diameter
+    ky, kz = 200, 300
+    zc = (gso + D * kz) - H               # z + H' = the layer's centre (no shift)
+    x = [np.nextafter(H, 0.0)]
+    y = [gso + D * ky]
+    z = [zc]
+    if partner:
+        x += [gso, 0.3]
+        y += [gso + D * (ky + 1), gso + D * ky + 1.0]
+        z += [zc, zc + 1.0]
+    x, y, z = (np.array(a, np.float64) for a in (x, y, z))
+    e = np.linspace(10.0, 20.0, len(x))
+    # the oracle's keys: the face point's key equals that of sphere (0, ky + 1, kz)
+    alias_key = (ky + 1) * N + kz * N * N
+    pk, _, keys, sums = po.track_segments(x, y, z, e, g, W.SEED, 0, 1, False, False)[0]
+    assert int(np.count_nonzero(pk == alias_key)) == (2 if partner else 1)
+    return g, x, y, z, e, alias_key, keys, sums
+
+
+@pytest.mark.parametrize("precision", [st.ST_F64, st.ST_F32])
+def test_batch_face_alias_merges_like_reference(precision):
+    """A point whose x index rounds to N shares the reference's key with sphere
+    (0, ky + 1, kz); when that sphere lies in the segment's key box, the two
+    points accumulate into one entry, with the oracle's key and sum."""
+    g, x, y, z, e, alias_key, keys, sums = _face_alias_track(True)
+    off = np.array([0, len(x)], np.int64)
+    oracle = {0: (keys, sums)}
+    counts, rec, info, edges = gpu_run(x, y, z, e, off, [(2.5e3, 10.0)], None, 1, W.SEED, False, False, precision,
+                                       [("log", 1000, 1e-2, 1e3)], record=64)
+    n, _ = compare_records(rec, oracle, 1e-12 if precision == st.ST_F64 else 1e-6)
+    assert n == len(keys) and alias_key in set(int(k) for k in rec[1])
+    check_bins(counts, edges, oracle, 1e-12 if precision == st.ST_F64 else 1e-6)
+
+
+@pytest.mark.parametrize("precision", [st.ST_F64, st.ST_F32])
+def test_batch_face_alias_outside_key_box_is_reported(precision):
+    """The same face point alone: sphere (0, ky + 1, kz) lies outside the
+    segment's key box, which the 32-bit table keys cannot express; the pass
+    reports it (error code 4) instead of scoring it under another key."""
+    g, x, y, z, e, alias_key, keys, sums = _face_alias_track(False)
+    off = np.array([0, len(x)], np.int64)
+    with pytest.raises(st.SuperTrackError, match="code 4"):
+        gpu_run(x, y, z, e, off, [(2.5e3, 10.0)], None, 1, W.SEED, False, False, precision,
+                [("log", 1000, 1e-2, 1e3)])
