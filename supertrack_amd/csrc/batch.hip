@@ -283,7 +283,7 @@ struct Acc<ST_F32> {
 };
 
 // Home slot in [0, SLOTS) of a key: multiplicative (Fibonacci) hash scaled
-// to the table size.
+// to the table size (measured faster than the same hash in FP64).
 template <int SLOTS>
 __device__ __forceinline__ unsigned int slot_hash(unsigned key)
 {
@@ -327,20 +327,20 @@ __device__ int wave_lower_bound(int n, Pred pred)
 // the persistent fused kernel
 // ---------------------------------------------------------------------------
 // kPPT: library blocks (points per thread) loaded together per iteration.
-// The FP32 kernel with the 1536-slot table (31 KB of LDS) is asked to fit five
-// workgroups per CU: 20 waves instead of 16 hide more of the LDS-atomic and
-// load latency.
+// The FP32 kernel with tables of up to 1920 slots (with 1000 histogram bins,
+// at most 31 KB of LDS) is asked to fit five workgroups per CU: 20 waves
+// instead of 16 hide more of the LDS-atomic and load latency.
 template <int MODE, int SLOTS, int kPPT>
 constexpr int waves_per_eu()
 {
-    return (MODE == ST_F32 && (SLOTS == 1280 || SLOTS == 1536) && kPPT == 1) ? 5 : 1;
+    return (MODE == ST_F32 && (SLOTS == 1280 || SLOTS == 1536 || SLOTS == 1920) && kPPT == 1) ? 5 : 1;
 }
 
 template <int MODE, int SLOTS, int kPPT>
 __global__ __launch_bounds__(kBT) __attribute__((amdgpu_waves_per_eu(waves_per_eu<MODE, SLOTS, kPPT>())))
 void batch_kernel(BatchArgs a)
 {
-    static_assert(SLOTS == 1280 || SLOTS == 1536 || SLOTS == 2048 || SLOTS == 4096, "table sizes");
+    static_assert(SLOTS == 1280 || SLOTS == 1536 || SLOTS == 1920 || SLOTS == 2048 || SLOTS == 4096, "table sizes");
     // occupancy never exceeds max(half table, survivors) + one iteration's points
     // A flush starts once the table is a quarter full: a low load factor
     // keeps the linear-probe chains (dependent LDS round trips, paid by the
@@ -349,7 +349,7 @@ void batch_kernel(BatchArgs a)
     constexpr int kFlushAt = SLOTS / 4;
     // occupancy never exceeds max(flush threshold, survivors) + one iteration's points
     // iterations between occupancy checks (each check is a workgroup barrier)
-    constexpr int kCheck = (SLOTS == 1536 && kPPT == 1) ? 2 : 1;
+    constexpr int kCheck = ((SLOTS == 1536 || SLOTS == 1920) && kPPT == 1) ? 2 : 1;
     static_assert((kFlushAt > kMaxWindow ? kFlushAt : kMaxWindow) + kCheck * kPPT * kBlk <= SLOTS * 9 / 10,
                   "the table must hold the survivors of a flush plus an iteration");
     // entries at a flush: those kept by the previous flush (at most kMaxWindow)
@@ -378,6 +378,7 @@ void batch_kernel(BatchArgs a)
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
+    const int wave_first = uniform_i32(tid & ~63);   // first thread of this wave
     // per-thread statistics, summed at exit
     unsigned int n_scored = 0, n_flagged = 0, n_in = 0;
     // conservation: energy words into and out of the table, entries created
@@ -423,7 +424,9 @@ void batch_kernel(BatchArgs a)
         if (new_entries) n_created += (unsigned long long)cnt;
         int base = 0;
         if (lane == 0) base = atomicAdd(&s_occ, cnt);
-        base = __shfl(base, 0, 64);
+        // lane 0 is active here (every lane calls append): a scalar read
+        // of its value, not a cross-lane LDS permute
+        base = __builtin_amdgcn_readfirstlane(base);
 #pragma unroll
         for (int j = 0; j < kN; ++j) {
             const int below = __builtin_amdgcn_mbcnt_hi((unsigned)(m[j] >> 32),
@@ -529,34 +532,51 @@ void batch_kernel(BatchArgs a)
         if (tid == 0) s_occ = 0;
         my_cnt = 0;
         __syncthreads();
-        int placed[kLP];
+        // entries walked by this wave's lanes (rows of 64 list positions
+        // from wave_first, n uniform: scalar arithmetic)
+        unsigned long long walked = 0;
 #pragma unroll
-        for (int r = 0; r < kLP; ++r) {
-            placed[r] = -1;
-            if (kk[r] != kEmpty32) {
-                unsigned int h = slot_hash<SLOTS>(kk[r]);
+        for (int r = 0; r < kLP; ++r) walked += (unsigned long long)min(max(n - (r * kBT + wave_first), 0), 64);
+        // the few incomplete entries (a z-window of spheres) are re-inserted
+        // into the cleared table; most waves have none and skip the step
+        bool keep = false;
+#pragma unroll
+        for (int r = 0; r < kLP; ++r) keep |= kk[r] != kEmpty32;
+        if (__ballot(keep) != 0ull) {
+            // first probes of every row at once (independent exchanges, one
+            // wait), then the rare collisions slot by slot; keys are unique,
+            // so a probe ends at the first empty slot
+            int placed[kLP];
+            unsigned prev[kLP];
+#pragma unroll
+            for (int r = 0; r < kLP; ++r) {
+                placed[r] = -1;
+                prev[r] = kEmpty32;
+                if (kk[r] != kEmpty32) {
+                    placed[r] = (int)slot_hash<SLOTS>(kk[r]);
+                    prev[r] = atomicCAS(&tkey[placed[r]], kEmpty32, kk[r]);
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < kLP; ++r) {
+                if (kk[r] == kEmpty32) continue;
+                unsigned int h = (unsigned)placed[r];
 #pragma unroll 1
-                for (int probe = 0; probe < SLOTS; ++probe) {
-                    const unsigned prev = atomicCAS(&tkey[h], kEmpty32, kk[r]);
-                    if (prev == kEmpty32) break;
+                for (int probe = 1; probe < SLOTS && prev[r] != kEmpty32; ++probe) {
                     h = next_slot<SLOTS>(h);
+                    prev[r] = atomicCAS(&tkey[h], kEmpty32, kk[r]);
                 }
                 placed[r] = (int)h;
                 atomicAdd(&tacc0[h], k0[r]);
                 if (kWords == 2) atomicAdd(&tacc1[h], k1[r]);
             }
-        }
-        // entries walked by this wave's lanes less those it placed again
-        {
-            unsigned long long walked = 0, kept = 0;
+            unsigned long long kept = 0;
 #pragma unroll
-            for (int r = 0; r < kLP; ++r) {
-                walked += __popcll(__ballot(tid + r * kBT < n));
-                kept += __popcll(__ballot(placed[r] >= 0));
-            }
-            n_retired += walked - kept;
+            for (int r = 0; r < kLP; ++r) kept += __popcll(__ballot(placed[r] >= 0));
+            walked -= kept;
+            append(placed, std::integral_constant<int, kLP>(), false);
         }
-        append(placed, std::integral_constant<int, kLP>(), false);
+        n_retired += walked;
         __syncthreads();
         __builtin_amdgcn_s_setprio(0);
     };
@@ -656,18 +676,20 @@ void batch_kernel(BatchArgs a)
             const double zhi = fmin(g.H, g.gso + ((double)k1 + 1.0) * g.D);
             // block range: blocks whose z span can reach inside (zlo, zhi)
             __builtin_amdgcn_s_setprio(2);
-            if (tid < 64) {
+            // (two waves search at once: the ends of the range)
+            if (tid < 128) {
                 const double *zf = a.zfirst + td.block0;
                 const int nb = td.nblocks;
                 auto zs_of = [&](double z) { return (z + g.H) + sh.z; };
-                const int r_lo = wave_lower_bound(nb, [&](int b) {
-                    const double zhb = (b + 1 < nb) ? zf[b + 1] : td.zmax;
-                    return zs_of(zhb) > zlo;
-                });
-                const int r_hi = wave_lower_bound(nb, [&](int b) { return !(zs_of(zf[b]) < zhi); });
-                if (tid == 0) {
-                    s_range[0] = r_lo;
-                    s_range[1] = r_hi;
+                if (tid < 64) {
+                    const int r_lo = wave_lower_bound(nb, [&](int b) {
+                        const double zhb = (b + 1 < nb) ? zf[b + 1] : td.zmax;
+                        return zs_of(zhb) > zlo;
+                    });
+                    if (tid == 0) s_range[0] = r_lo;
+                } else {
+                    const int r_hi = wave_lower_bound(nb, [&](int b) { return !(zs_of(zf[b]) < zhi); });
+                    if (tid == 64) s_range[1] = r_hi;
                 }
             }
             __syncthreads();
@@ -700,6 +722,10 @@ void batch_kernel(BatchArgs a)
 #pragma unroll
                     for (int j = 0; j < kPPT; ++j) cur[j] = nxt[j];
                     if (b + kPPT < b_hi) fetch(b + kPPT, nxt);
+                    // z of the next iteration's first point (its block's first
+                    // point), requested now through the scalar cache so that a
+                    // flush does not wait on it
+                    const double z_next = load_uniform(a.zfirst + td.block0 + min(b + kPPT, td.nblocks - 1));
                     const int64_t c = td.offset + (int64_t)b * kBlk;
                     int fresh[kPPT];
 #pragma unroll
@@ -807,7 +833,7 @@ void batch_kernel(BatchArgs a)
                         // layer index (z order), and a key's layer is never below its
                         // point's layer: entries in the slab's layers below it are complete
                         // (the next point lies in the range: |its layer| < 2^30)
-                        const double zs = (a.z64[next] + g.H) + s_shift[2];
+                        const double zs = (z_next + g.H) + s_shift[2];
                         const int lb = uniform_i32((int)nearest_index(zs - g.gso, g.D, g.invD)) - (int)box.kz0;
                         unsigned bound = 0u;
                         if (lb >= (int)box.nz) bound = kEmpty32;
@@ -890,8 +916,10 @@ const void *kernel_ptr()
 template <int MODE>
 const void *pick_kernel_mode(int slots, int ppt)
 {
-    // 1280 slots hold the largest survivor set plus one point per thread only
+    // 1280 slots hold the largest survivor set plus one point per thread only;
+    // 1920 slots are built for one point per thread only
     if (slots == 1280) return ppt == 2 ? nullptr : kernel_ptr<MODE, 1280, 1>();
+    if (slots == 1920) return ppt == 2 ? nullptr : kernel_ptr<MODE, 1920, 1>();
     if (slots == 1536) return ppt == 2 ? kernel_ptr<MODE, 1536, 2>() : kernel_ptr<MODE, 1536, 1>();
     if (slots == 2048) return ppt == 2 ? kernel_ptr<MODE, 2048, 2>() : kernel_ptr<MODE, 2048, 1>();
     return ppt == 2 ? kernel_ptr<MODE, 4096, 2>() : kernel_ptr<MODE, 4096, 1>();
@@ -1115,8 +1143,8 @@ int default_launch(int precision, int hist_words, const st_batch_opts &o, Launch
 {
     const bool f32 = precision == ST_F32;
     const int slots = o.table_slots ? o.table_slots : 1536;
-    if (slots != 1280 && slots != 1536 && slots != 2048 && slots != 4096)
-        return set_error(ST_EINVAL, "batched run: table_slots must be 1280, 1536, 2048 or 4096");
+    if (slots != 1280 && slots != 1536 && slots != 1920 && slots != 2048 && slots != 4096)
+        return set_error(ST_EINVAL, "batched run: table_slots must be 1280, 1536, 1920, 2048 or 4096");
     const int bpc = o.blocks_per_cu ? o.blocks_per_cu : (f32 ? 5 : 3);
     const int ppt = o.points_per_thread ? o.points_per_thread : (f32 ? 1 : 2);
     if (ppt != 1 && ppt != 2) return set_error(ST_EINVAL, "batched run: points_per_thread must be 1 or 2");
@@ -1209,7 +1237,7 @@ int autotune(st_batch *b)
                                   hipMemcpyDeviceToHost, b->stream));
     }
     ST_HIP_TRY(hipStreamSynchronize(b->stream));
-    const int cand_slots[] = {1280, 1536, 2048, 4096};
+    const int cand_slots[] = {1280, 1536, 1920, 2048, 4096};
     const int cand_bpc[] = {2, 3, 4, 5, 6, 8};
     const int cand_copies[] = {1, 2, 4};
     float best = 1e30f;
