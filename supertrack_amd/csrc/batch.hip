@@ -369,12 +369,14 @@ void batch_kernel(BatchArgs a)
     __shared__ long long s_seg;
     __shared__ int s_occ;
     __shared__ int s_range[2];
-    // the segment's shift (nm), read only by the rare exact path and at
-    // flushes, so that it occupies no registers through the point loop
+    // the segment's shift (nm), read only by the rare exact path, so that it
+    // occupies no registers through the point loop
     __shared__ double s_shift[3];
-    // entries each wave listed since the last flush, published at each
-    // occupancy check into alternating halves
+    // entries each wave listed since the last flush, and a lower bound on the
+    // layer of every later point, published at each occupancy check into
+    // alternating halves
     __shared__ int s_wcnt[2][kBT / 64];
+    __shared__ int s_layer[2];
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -722,11 +724,10 @@ void batch_kernel(BatchArgs a)
 #pragma unroll
                     for (int j = 0; j < kPPT; ++j) cur[j] = nxt[j];
                     if (b + kPPT < b_hi) fetch(b + kPPT, nxt);
-                    // z of the next iteration's first point (its block's first
-                    // point), requested now through the scalar cache so that a
-                    // flush does not wait on it
-                    const double z_next = load_uniform(a.zfirst + td.block0 + min(b + kPPT, td.nblocks - 1));
                     const int64_t c = td.offset + (int64_t)b * kBlk;
+                    // the iteration's last point (thread kBT - 1, last row):
+                    // its local layer, less one if its rounding was uncertain
+                    int last_layer = 0;
                     int fresh[kPPT];
 #pragma unroll
                     for (int j = 0; j < kPPT; ++j) fresh[j] = -1;
@@ -771,6 +772,7 @@ void batch_kernel(BatchArgs a)
                             (fmax(fmax(fabs(ux), fabs(uy)), fabs(uz)) >= fm) | (fabs(distq - g.r2q) <= mrgq);
                         in = valid & inbox & !near & (distq < g.r2q);
                         key = (lz * box.ny + ly) * box.nx + lx;
+                        if (j == kPPT - 1) last_layer = (int)lz - (int)near;
                         if (valid & near) {
                             // any decision within the error bound is redone exactly
                             // from FP64, unless clearly outside (an index beyond the
@@ -823,18 +825,21 @@ void batch_kernel(BatchArgs a)
                     // counts read here cannot change under a slower wave (which would
                     // otherwise flush alone).  The sum equals s_occ at the barrier.
                     if (lane == 0) s_wcnt[par][tid >> 6] = my_cnt;
+                    if (tid == kBT - 1) s_layer[par] = last_layer;
                     __syncthreads();
                     int occ = 0;
 #pragma unroll
                     for (int w = 0; w < kBT / 64; ++w) occ += s_wcnt[par][w];
+                    const int lb = uniform_i32(s_layer[par]);
                     par ^= 1;
                     if (occ >= kFlushAt) {
-                        // every later point lies in a layer >= the next point's exact
-                        // layer index (z order), and a key's layer is never below its
-                        // point's layer: entries in the slab's layers below it are complete
-                        // (the next point lies in the range: |its layer| < 2^30)
-                        const double zs = (z_next + g.H) + s_shift[2];
-                        const int lb = uniform_i32((int)nearest_index(zs - g.gso, g.D, g.invD)) - (int)box.kz0;
+                        // The exact layer index is monotone in z, so every later
+                        // point lies in a layer >= that of this iteration's last
+                        // point (the block is full: the range goes on), which is
+                        // its rounded local layer unless that rounding was within
+                        // the error bound (then one less); a key's layer is never
+                        // below its point's: entries in the slab's layers below
+                        // lb are complete.
                         unsigned bound = 0u;
                         if (lb >= (int)box.nz) bound = kEmpty32;
                         else if (lb > 0) bound = (unsigned)lb * box.nxy;
