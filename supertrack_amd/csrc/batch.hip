@@ -215,15 +215,27 @@ __device__ __forceinline__ double nearest_index(double a, double D, double invD)
     return k;
 }
 
+// What the exact path needs of the current segment and slab.  The kernel
+// keeps it in LDS: the path is rare, and its constants then hold no
+// registers through the point loop.
+struct ExactParams {
+    double H, gso, D, invD, r2;       // geometry
+    double sx, sy, sz;                // shift
+    unsigned N;
+    unsigned kx0, ky0, nx, ny;        // the segment's key box in x and y
+    unsigned kz_lo, nz_all, multi;    // its layers, and whether it is scored in slabs
+    unsigned kz0, nz;                 // the current slab's layers
+};
+
 // On success kx, ky, kz are the reference's sphere indices, in [0, N]: an
 // index equals N only for a point within rounding of the voxel's upper face.
-__device__ __forceinline__ bool exact_member(double x, double y, double zp, const GeomDev &g,
-                                             double sx, double sy, double sz, unsigned &ix, unsigned &iy,
-                                             unsigned &iz)
+// z: the stored coordinate (H' is added here, as the reference adds it once).
+__device__ __forceinline__ bool exact_member(double x, double y, double z, const ExactParams &g, unsigned &ix,
+                                             unsigned &iy, unsigned &iz)
 {
-    const double xs = x + sx;
-    const double ys = y + sy;
-    const double zs = zp + sz;
+    const double xs = x + g.sx;
+    const double ys = y + g.sy;
+    const double zs = (z + g.H) + g.sz;
     if (!(fabs(xs) < g.H && fabs(ys) < g.H && fabs(zs) < g.H)) return false;
     const double ax = xs - g.gso, ay = ys - g.gso, az = zs - g.gso;
     const double kx = nearest_index(ax, g.D, g.invD);
@@ -369,9 +381,8 @@ void batch_kernel(BatchArgs a)
     __shared__ long long s_seg;
     __shared__ int s_occ;
     __shared__ int s_range[2];
-    // the segment's shift (nm), read only by the rare exact path, so that it
-    // occupies no registers through the point loop
-    __shared__ double s_shift[3];
+    // the exact path's constants (read only by that rare path)
+    __shared__ ExactParams s_ex;
     // entries each wave listed since the last flush, and a lower bound on the
     // layer of every later point, published at each occupancy check into
     // alternating halves
@@ -636,38 +647,58 @@ void batch_kernel(BatchArgs a)
         // upper face) is carried into the next index, giving the indices of
         // the sphere whose key the reference's sum kx + N ky + N^2 kz names;
         // both then accumulate into that one entry, as in the reference.
-        auto exact_local = [&](double x, double y, double zp, unsigned &key) -> bool {
+        // (all from s_ex, the exact path's constants in LDS)
+        auto exact_local = [&](double x, double y, double z, unsigned &key) -> bool {
+            const ExactParams &e = s_ex;
             unsigned gx, gy, gz;
-            if (!exact_member(x, y, zp, g, s_shift[0], s_shift[1], s_shift[2], gx, gy, gz)) return false;
-            const bool alias = max(max(gx, gy), gz) >= N;
-            if (gx >= N) {
-                gx -= N;
+            if (!exact_member(x, y, z, e, gx, gy, gz)) return false;
+            const bool alias = max(max(gx, gy), gz) >= e.N;
+            if (gx >= e.N) {
+                gx -= e.N;
                 ++gy;
             }
-            if (gy >= N) {
-                gy -= N;
+            if (gy >= e.N) {
+                gy -= e.N;
                 ++gz;
             }
-            const unsigned lx = gx - box.kx0, ly = gy - box.ky0, lz = gz - box.kz0;
+            const unsigned lx = gx - e.kx0, ly = gy - e.ky0, lz = gz - e.kz0;
             // every in-sphere point lies in the segment's box by construction;
             // only a carried index can leave it (or, across slabs, move to
             // another slab): reported, never scored wrong
-            if (lx >= box.nx || ly >= box.ny || gz - kz_lo >= nz_all || (alias && multi))
+            if (lx >= e.nx || ly >= e.ny || gz - e.kz_lo >= e.nz_all || (alias && e.multi))
                 atomicOr(&a.ctl->error, 4);
-            if (!(lx < box.nx && ly < box.ny && lz < box.nz)) return false;
-            key = (lz * box.ny + ly) * box.nx + lx;
+            if (!(lx < e.nx && ly < e.ny && lz < e.nz)) return false;
+            key = (lz * e.ny + ly) * e.nx + lx;
             return true;
         };
         // (read after the block-range barrier below; rewritten only after
-        // the next segment's ticket barriers)
+        // the next segment's ticket barriers, or, for a slab's layers, after
+        // the previous slab's last flush)
         if (tid == 0) {
-            s_shift[0] = sh.x;
-            s_shift[1] = sh.y;
-            s_shift[2] = sh.z;
+            s_ex.H = g.H;
+            s_ex.gso = g.gso;
+            s_ex.D = g.D;
+            s_ex.invD = g.invD;
+            s_ex.r2 = g.r2;
+            s_ex.sx = sh.x;
+            s_ex.sy = sh.y;
+            s_ex.sz = sh.z;
+            s_ex.N = N;
+            s_ex.kx0 = kx0;
+            s_ex.ky0 = ky0;
+            s_ex.nx = nx;
+            s_ex.ny = ny;
+            s_ex.kz_lo = kz_lo;
+            s_ex.nz_all = nz_all;
+            s_ex.multi = multi ? 1u : 0u;
         }
         for (unsigned k0 = kz_lo;;) {
             const unsigned k1 = min(kz_hi, k0 + layers - 1u);
             box = SlabBox{kx0, ky0, k0, nx, ny, k1 - k0 + 1u, (unsigned)nxy};
+            if (tid == 0) {
+                s_ex.kz0 = k0;
+                s_ex.nz = box.nz;
+            }
             // lattice-unit offsets that make the rounded coordinates local indices
             const double ox = (MODE == ST_F64 ? sqx - g.qc : sqx) - (double)kx0;
             const double oy = (MODE == ST_F64 ? sqy - g.qc : sqy) - (double)ky0;
@@ -782,9 +813,9 @@ void batch_kernel(BatchArgs a)
                             // (statistic of the FP32 path: decisions sent to FP64)
                             if constexpr (MODE == ST_F32) n_flagged += cand;
                             if constexpr (MODE == ST_F64)
-                                in = cand && exact_local(cur[j].x, cur[j].y, cur[j].z + g.H, key);
+                                in = cand && exact_local(cur[j].x, cur[j].y, cur[j].z, key);
                             else
-                                in = cand && exact_local(a.x64[i], a.y64[i], a.z64[i] + g.H, key);
+                                in = cand && exact_local(a.x64[i], a.y64[i], a.z64[i], key);
                         }
                         if constexpr (MODE == ST_F64) {
                             if (in) Acc<ST_F64>::split(cur[j].w / g.denom, v0, v1);
