@@ -122,7 +122,7 @@ struct BatchArgs {
 
     // recording (test hook)
     int64_t rec_cap;
-    int64_t *rec_seg;
+    int64_t *rec_seg;              // the segment's ticket
     unsigned long long *rec_key;   // the local key
     unsigned long long *rec_box;   // 3 words per record: kx0 | ky0 << 32, kz0 | nx << 32, ny
     double *rec_sum;
@@ -499,9 +499,9 @@ void batch_kernel(BatchArgs a)
                     if (em[u] && a.rec_cap > 0) {
                         const unsigned long long p = atomicAdd(&a.ctl->rec_count, 1ull);
                         if ((int64_t)p < a.rec_cap) {
-                            // test hook only: the segment id from the ticket
-                            const long long tk = s_seg;
-                            a.rec_seg[p] = (int64_t)a.order[tk / a.n_os] * a.n_os + tk % a.n_os;
+                            // test hook only: the ticket (the host maps it to
+                            // the segment id)
+                            a.rec_seg[p] = (int64_t)s_seg;
                             a.rec_key[p] = k;
                             a.rec_box[3 * p] = box.kx0 | ((unsigned long long)box.ky0 << 32);
                             a.rec_box[3 * p + 1] = box.kz0 | ((unsigned long long)box.nx << 32);
@@ -1785,15 +1785,19 @@ int st_batch_record_fetch(st_batch *b, int64_t *segments, uint64_t *keys, double
     *count = (int64_t)c;
     if ((int64_t)c > b->rec_cap) return set_error(ST_ERANGE, "st_batch_record_fetch: %llu records, capacity %lld",
                                                  c, (long long)b->rec_cap);
-    if (c && segments) ST_HIP_TRY(hipMemcpy(segments, b->rec_seg, c * sizeof(int64_t), hipMemcpyDeviceToHost));
+    // segment id (track * n_os + oversample) from the ticket: the ticket
+    // counts segments in the order of the fast tracks
+    const int64_t n_os = std::max(b->opts.n_oversamples, 1);
+    std::vector<int64_t> seg(c);
+    if (c) ST_HIP_TRY(hipMemcpy(seg.data(), b->rec_seg, c * sizeof(int64_t), hipMemcpyDeviceToHost));
+    for (unsigned long long i = 0; i < c; ++i)
+        seg[i] = (int64_t)b->lib->fast_order[seg[i] / n_os] * n_os + seg[i] % n_os;
+    if (c && segments) std::memcpy(segments, seg.data(), c * sizeof(int64_t));
     if (c && keys) {
         // the reference's key from the local key and its slab's box
-        std::vector<int64_t> seg(c);
         std::vector<unsigned long long> loc(c), bx(3 * c);
-        ST_HIP_TRY(hipMemcpy(seg.data(), b->rec_seg, c * sizeof(int64_t), hipMemcpyDeviceToHost));
         ST_HIP_TRY(hipMemcpy(loc.data(), b->rec_key, c * sizeof(uint64_t), hipMemcpyDeviceToHost));
         ST_HIP_TRY(hipMemcpy(bx.data(), b->rec_box, 3 * c * sizeof(uint64_t), hipMemcpyDeviceToHost));
-        const int64_t n_os = std::max(b->opts.n_oversamples, 1);
         for (unsigned long long i = 0; i < c; ++i) {
             const TrackDesc &td = b->lib->tracks[seg[i] / n_os];
             const unsigned long long N = (unsigned long long)b->lib->geoms[td.geom].n_linear;
