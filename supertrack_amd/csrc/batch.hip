@@ -121,11 +121,17 @@ struct BatchArgs {
     int hist_copies;        // LDS copies of the histograms (wave w counts into copy w % copies)
 
     // recording (test hook)
-    int64_t rec_cap;
-    int64_t *rec_seg;              // the segment's ticket
-    unsigned long long *rec_key;   // the local key
-    unsigned long long *rec_box;   // 3 words per record: kx0 | ky0 << 32, kz0 | nx << 32, ny
-    double *rec_sum;
+    const struct RecArgs *rec;     // null: off
+};
+
+// Recording of every binned sphere (test hook), in device memory so that the
+// kernel keeps one pointer, not five, through its loops.
+struct RecArgs {
+    int64_t cap;
+    int64_t *seg;                  // the segment's ticket
+    unsigned long long *key;       // the local key
+    unsigned long long *box;       // 3 words per record: kx0 | ky0 << 32, kz0 | nx << 32, ny
+    double *sum;
 };
 
 __device__ __forceinline__ int uniform_i32(int v)
@@ -496,17 +502,18 @@ void batch_kernel(BatchArgs a)
                     k1[r] = v1;
                     vv[u] = Acc<MODE>::value(v0, v1);
                     if (em[u]) bal_out += Acc<MODE>::balance(v0, v1);
-                    if (em[u] && a.rec_cap > 0) {
+                    if (em[u] && a.rec != nullptr) {
+                        const RecArgs rc = load_uniform(a.rec);
                         const unsigned long long p = atomicAdd(&a.ctl->rec_count, 1ull);
-                        if ((int64_t)p < a.rec_cap) {
+                        if ((int64_t)p < rc.cap) {
                             // test hook only: the ticket (the host maps it to
                             // the segment id)
-                            a.rec_seg[p] = (int64_t)s_seg;
-                            a.rec_key[p] = k;
-                            a.rec_box[3 * p] = box.kx0 | ((unsigned long long)box.ky0 << 32);
-                            a.rec_box[3 * p + 1] = box.kz0 | ((unsigned long long)box.nx << 32);
-                            a.rec_box[3 * p + 2] = box.ny;
-                            a.rec_sum[p] = vv[u];
+                            rc.seg[p] = (int64_t)s_seg;
+                            rc.key[p] = k;
+                            rc.box[3 * p] = s_ex.kx0 | ((unsigned long long)s_ex.ky0 << 32);
+                            rc.box[3 * p + 1] = s_ex.kz0 | ((unsigned long long)s_ex.nx << 32);
+                            rc.box[3 * p + 2] = s_ex.ny;
+                            rc.sum[p] = vv[u];
                         }
                     }
                 }
@@ -1055,11 +1062,8 @@ struct st_batch {
     Launch ln;
     int hist_words = 0;
     BatchCtl *ctl = nullptr;   // per-pass control block (device)
-    int64_t rec_cap = 0;
-    int64_t *rec_seg = nullptr;
-    unsigned long long *rec_key = nullptr;
-    unsigned long long *rec_box = nullptr;
-    double *rec_sum = nullptr;
+    RecArgs rec{};                // recording buffers (cap 0: off)
+    RecArgs *rec_dev = nullptr;   // their descriptor in device memory
     bool graph_ready = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -1125,11 +1129,7 @@ int launch_pass(int precision, const BatchArgs &args, const Launch &ln, hipStrea
 int launch_fast(st_batch *b)
 {
     BatchArgs a = make_args(b->lib, b->opts, b->hists.data(), (int)b->hists.size(), b->ln, b->ctl);
-    a.rec_cap = b->rec_cap;
-    a.rec_seg = b->rec_seg;
-    a.rec_key = b->rec_key;
-    a.rec_box = b->rec_box;
-    a.rec_sum = b->rec_sum;
+    a.rec = b->rec.cap > 0 ? b->rec_dev : nullptr;
     return launch_pass(b->lib->precision, a, b->ln, b->stream);
 }
 
@@ -1680,7 +1680,7 @@ int st_batch_run(st_batch *b)
     if (!b->ev[0])
         for (hipEvent_t &e : b->ev) ST_HIP_TRY(hipEventCreate(&e));
     ST_HIP_TRY(hipEventRecord(b->ev[0], b->stream));
-    if (b->opts.use_graph && b->rec_cap == 0) {
+    if (b->opts.use_graph && b->rec.cap == 0) {
         if (!b->graph_ready) {
             ST_HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
             int rc = launch_fast(b);
@@ -1757,21 +1757,21 @@ int st_batch_params(const st_batch *b, int32_t *p)
 int st_batch_record(st_batch *b, int64_t capacity)
 {
     if (!b || capacity < 0) return set_error(ST_EINVAL, "st_batch_record: bad arguments");
-    (void)hipFree(b->rec_seg);
-    (void)hipFree(b->rec_key);
-    (void)hipFree(b->rec_box);
-    (void)hipFree(b->rec_sum);
-    b->rec_seg = nullptr;
-    b->rec_key = nullptr;
-    b->rec_box = nullptr;
-    b->rec_sum = nullptr;
-    b->rec_cap = 0;
+    (void)hipFree(b->rec.seg);
+    (void)hipFree(b->rec.key);
+    (void)hipFree(b->rec.box);
+    (void)hipFree(b->rec.sum);
+    (void)hipFree(b->rec_dev);
+    b->rec = RecArgs{};
+    b->rec_dev = nullptr;
     if (capacity == 0) return ST_OK;
-    ST_ALLOC_TRY(dev_malloc((void **)&b->rec_seg, capacity * sizeof(int64_t)));
-    ST_ALLOC_TRY(dev_malloc((void **)&b->rec_key, capacity * sizeof(unsigned long long)));
-    ST_ALLOC_TRY(dev_malloc((void **)&b->rec_box, 3 * capacity * sizeof(unsigned long long)));
-    ST_ALLOC_TRY(dev_malloc((void **)&b->rec_sum, capacity * sizeof(double)));
-    b->rec_cap = capacity;
+    ST_ALLOC_TRY(dev_malloc((void **)&b->rec.seg, capacity * sizeof(int64_t)));
+    ST_ALLOC_TRY(dev_malloc((void **)&b->rec.key, capacity * sizeof(unsigned long long)));
+    ST_ALLOC_TRY(dev_malloc((void **)&b->rec.box, 3 * capacity * sizeof(unsigned long long)));
+    ST_ALLOC_TRY(dev_malloc((void **)&b->rec.sum, capacity * sizeof(double)));
+    ST_ALLOC_TRY(dev_malloc((void **)&b->rec_dev, sizeof(RecArgs)));
+    b->rec.cap = capacity;
+    ST_HIP_TRY(hipMemcpy(b->rec_dev, &b->rec, sizeof(RecArgs), hipMemcpyHostToDevice));
     return ST_OK;
 }
 
@@ -1783,21 +1783,21 @@ int st_batch_record_fetch(st_batch *b, int64_t *segments, uint64_t *keys, double
     if (rc) return rc;
     const unsigned long long c = ctl.rec_count;
     *count = (int64_t)c;
-    if ((int64_t)c > b->rec_cap) return set_error(ST_ERANGE, "st_batch_record_fetch: %llu records, capacity %lld",
-                                                 c, (long long)b->rec_cap);
+    if ((int64_t)c > b->rec.cap) return set_error(ST_ERANGE, "st_batch_record_fetch: %llu records, capacity %lld",
+                                                 c, (long long)b->rec.cap);
     // segment id (track * n_os + oversample) from the ticket: the ticket
     // counts segments in the order of the fast tracks
     const int64_t n_os = std::max(b->opts.n_oversamples, 1);
     std::vector<int64_t> seg(c);
-    if (c) ST_HIP_TRY(hipMemcpy(seg.data(), b->rec_seg, c * sizeof(int64_t), hipMemcpyDeviceToHost));
+    if (c) ST_HIP_TRY(hipMemcpy(seg.data(), b->rec.seg, c * sizeof(int64_t), hipMemcpyDeviceToHost));
     for (unsigned long long i = 0; i < c; ++i)
         seg[i] = (int64_t)b->lib->fast_order[seg[i] / n_os] * n_os + seg[i] % n_os;
     if (c && segments) std::memcpy(segments, seg.data(), c * sizeof(int64_t));
     if (c && keys) {
         // the reference's key from the local key and its slab's box
         std::vector<unsigned long long> loc(c), bx(3 * c);
-        ST_HIP_TRY(hipMemcpy(loc.data(), b->rec_key, c * sizeof(uint64_t), hipMemcpyDeviceToHost));
-        ST_HIP_TRY(hipMemcpy(bx.data(), b->rec_box, 3 * c * sizeof(uint64_t), hipMemcpyDeviceToHost));
+        ST_HIP_TRY(hipMemcpy(loc.data(), b->rec.key, c * sizeof(uint64_t), hipMemcpyDeviceToHost));
+        ST_HIP_TRY(hipMemcpy(bx.data(), b->rec.box, 3 * c * sizeof(uint64_t), hipMemcpyDeviceToHost));
         for (unsigned long long i = 0; i < c; ++i) {
             const TrackDesc &td = b->lib->tracks[seg[i] / n_os];
             const unsigned long long N = (unsigned long long)b->lib->geoms[td.geom].n_linear;
@@ -1808,7 +1808,7 @@ int st_batch_record_fetch(st_batch *b, int64_t *segments, uint64_t *keys, double
             keys[i] = (kx0 + lx) + (ky0 + ly) * N + (kz0 + lz) * N * N;
         }
     }
-    if (c && sums) ST_HIP_TRY(hipMemcpy(sums, b->rec_sum, c * sizeof(double), hipMemcpyDeviceToHost));
+    if (c && sums) ST_HIP_TRY(hipMemcpy(sums, b->rec.sum, c * sizeof(double), hipMemcpyDeviceToHost));
     return ST_OK;
 }
 
@@ -1823,7 +1823,7 @@ void st_batch_destroy(st_batch *b)
         if (e) (void)hipEventDestroy(e);
     st_track_free(&b->scratch);
     st_pairs_free(&b->pairs);
-    void *ptrs[] = {b->ctl, b->rec_seg, b->rec_key, b->rec_box, b->rec_sum};
+    void *ptrs[] = {b->ctl, b->rec.seg, b->rec.key, b->rec.box, b->rec.sum, b->rec_dev};
     for (void *p : ptrs) (void)hipFree(p);
     delete b;
 }
