@@ -200,6 +200,7 @@ __device__ __forceinline__ unsigned long long lattice_key(unsigned kx, unsigned 
 // is the same test in both.  A segment whose box exceeds 2^32 - 2 spheres is
 // scored in slabs of layers, each with its own box and a full flush at its end.
 constexpr unsigned kEmpty32 = 0xFFFFFFFFu;
+constexpr int kLayerUnsure = -0x7FFFFFFF;   // a layer to recompute exactly (|layer| < 2^30 otherwise)
 constexpr unsigned long long kMaxKeys = 0xFFFFFFFEull;   // keys per slab box
 
 struct SlabBox {
@@ -764,7 +765,8 @@ void batch_kernel(BatchArgs a)
                     if (b + kPPT < b_hi) fetch(b + kPPT, nxt);
                     const int64_t c = td.offset + (int64_t)b * kBlk;
                     // the iteration's last point (thread kBT - 1, last row):
-                    // its local layer, less one if its rounding was uncertain
+                    // its local layer, or kLayerUnsure if its rounding was
+                    // within the error bound
                     int last_layer = 0;
                     int fresh[kPPT];
 #pragma unroll
@@ -810,7 +812,7 @@ void batch_kernel(BatchArgs a)
                             (fmax(fmax(fabs(ux), fabs(uy)), fabs(uz)) >= fm) | (fabs(distq - g.r2q) <= mrgq);
                         in = valid & inbox & !near & (distq < g.r2q);
                         key = (lz * box.ny + ly) * box.nx + lx;
-                        if (j == kPPT - 1) last_layer = (int)lz - (int)near;
+                        if (j == kPPT - 1) last_layer = near ? kLayerUnsure : (int)lz;
                         if (valid & near) {
                             // any decision within the error bound is redone exactly
                             // from FP64, unless clearly outside (an index beyond the
@@ -873,14 +875,20 @@ void batch_kernel(BatchArgs a)
                     if (occ >= kFlushAt) {
                         // The exact layer index is monotone in z, so every later
                         // point lies in a layer >= that of this iteration's last
-                        // point (the block is full: the range goes on), which is
-                        // its rounded local layer unless that rounding was within
-                        // the error bound (then one less); a key's layer is never
-                        // below its point's: entries in the slab's layers below
-                        // lb are complete.
+                        // point (the block is full: the range goes on); a key's
+                        // layer is never below its point's: entries in the slab's
+                        // layers below lb are complete.
+                        int lb_exact = lb;
+                        if (lb == kLayerUnsure) {
+                            // rare: the exact layer, as the reference rounds it
+                            const double zz = load_uniform(a.z64 + (next - 1));
+                            const ExactParams &e = s_ex;
+                            lb_exact = uniform_i32((int)nearest_index(((zz + e.H) + e.sz) - e.gso, e.D, e.invD)) -
+                                       (int)box.kz0;
+                        }
                         unsigned bound = 0u;
-                        if (lb >= (int)box.nz) bound = kEmpty32;
-                        else if (lb > 0) bound = (unsigned)lb * box.nxy;
+                        if (lb_exact >= (int)box.nz) bound = kEmpty32;
+                        else if (lb_exact > 0) bound = (unsigned)lb_exact * box.nxy;
                         flush(bound, true);
                     }
                 }
