@@ -33,8 +33,9 @@ HBM_PEAK_GBS = 8000.0
 VALU_PEAK_GINSTR = 256 * 4 * 2.4e9 / 4 / 1e9
 # Kernel sources whose hash keys the committed counter summaries: counters
 # measured on other sources are not used.
-KERNEL_SOURCES = ["supertrack_amd/csrc/batch.hip", "supertrack_amd/csrc/st_common.hpp",
-                  "supertrack_amd/csrc/st_device.hpp", "supertrack_amd/csrc/st_internal.hpp"]
+KERNEL_SOURCES = ["supertrack_amd/csrc/batch_kernel.hpp", "supertrack_amd/csrc/st_common.hpp",
+                  "supertrack_amd/csrc/st_device.hpp", "supertrack_amd/csrc/st_internal.hpp",
+                  "supertrack_amd/csrc/layout.hpp"]
 
 
 def log(msg):
