@@ -65,11 +65,13 @@ const void *pick_kernel(int mode, int slots, int ppt)
     return mode == ST_F64 ? pick_kernel_mode<ST_F64>(slots, ppt) : pick_kernel_mode<ST_F32>(slots, ppt);
 }
 
-size_t lds_bytes(int mode, int slots, int hist_words)
+size_t lds_bytes(int mode, int slots, int ppt, int hist_words)
 {
     const int words = (mode == ST_F64) ? 2 : 1;
-    // per slot: the sums, the 32-bit key, the list entry
-    return (size_t)slots * (8 * words + 4 + 2) + (size_t)hist_words * 4;
+    // per slot: the sums and the 32-bit key; the waves' parts of the slot
+    // list; the privatised bins
+    return (size_t)slots * (8 * words + 4) + (size_t)(kBT / 64) * list_region(slots, ppt) * 2 +
+           (size_t)hist_words * 4;
 }
 
 template <typename T>
@@ -236,11 +238,11 @@ int configure_launch(int precision, int hist_words, int slots, int bpc, int ppt,
         return set_error(ST_EINVAL, "batched run: hist_copies must be 1, 2 or 4");
     // privatise the histograms in LDS when they fit next to the table at two
     // workgroups per CU; otherwise count straight into the device bins (one copy)
-    ln.lds_hist = lds_bytes(precision, slots, hist_words * copies) <= 80 * 1024;
+    ln.lds_hist = lds_bytes(precision, slots, ln.ppt, hist_words * copies) <= 80 * 1024;
     if (!ln.lds_hist && copies > 1)
         return set_error(ST_ERANGE, "batched run: %d LDS copies of %d histogram bins do not fit", copies, hist_words);
     ln.hist_copies = copies;
-    const size_t lds = lds_bytes(precision, slots, ln.lds_hist ? hist_words * copies : 0);
+    const size_t lds = lds_bytes(precision, slots, ln.ppt, ln.lds_hist ? hist_words * copies : 0);
     if (lds > 160 * 1024)
         return set_error(ST_ERANGE, "batched kernel needs %zu B of LDS (limit 160 KiB)", lds);
     const void *k = pick_kernel(precision, slots, ln.ppt);

@@ -322,6 +322,28 @@ __device__ int wave_lower_bound(int n, Pred pred)
     return lo;
 }
 
+// Occupancy checks happen every kCheck iterations; a flush starts at a
+// quarter-full table.
+constexpr int check_every(int slots, int ppt)
+{
+    return ((slots == 1536 || slots == 1920) && ppt == 1) ? 2 : 1;
+}
+
+// Capacity of each wave's part of the slot list.  A wave lists at most the
+// entries it walked in the last flush (kLP rows of 64: 64 * kLP) or, at an
+// occupancy check that did not flush, fewer than the flush threshold; then
+// at most the points of the iterations until the next check.
+constexpr int list_region(int slots, int ppt)
+{
+    const int flush_at = slots / 4;
+    const int checks = check_every(slots, ppt);
+    const int window = (int)kMaxWindow;
+    const int max_entries = (flush_at - 1 > window ? flush_at - 1 : window) + checks * ppt * kBlk;
+    const int rows = (max_entries + kBT - 1) / kBT;
+    const int walked = 64 * rows < window ? 64 * rows : window;
+    return (flush_at - 1 > walked ? flush_at - 1 : walked) + checks * ppt * 64;
+}
+
 // ---------------------------------------------------------------------------
 // the persistent fused kernel
 // ---------------------------------------------------------------------------
@@ -348,7 +370,7 @@ void batch_kernel(BatchArgs a)
     constexpr int kFlushAt = SLOTS / 4;
     // occupancy never exceeds max(flush threshold, survivors) + one iteration's points
     // iterations between occupancy checks (each check is a workgroup barrier)
-    constexpr int kCheck = ((SLOTS == 1536 || SLOTS == 1920) && kPPT == 1) ? 2 : 1;
+    constexpr int kCheck = check_every(SLOTS, kPPT);
     static_assert((kFlushAt > kMaxWindow ? kFlushAt : kMaxWindow) + kCheck * kPPT * kBlk <= SLOTS * 9 / 10,
                   "the table must hold the survivors of a flush plus an iteration");
     // entries at a flush: those kept by the previous flush (at most kMaxWindow)
@@ -361,12 +383,12 @@ void batch_kernel(BatchArgs a)
     unsigned long long *tacc0 = lds;
     unsigned long long *tacc1 = tacc0 + SLOTS;  // used when kWords == 2
     unsigned int *tkey = reinterpret_cast<unsigned int *>(tacc0 + SLOTS * kWords);
-    // slots in use, in insertion order (s_occ entries)
+    // slots in use, in insertion order: one part of kRegion entries per wave
+    constexpr int kRegion = list_region(SLOTS, kPPT);
     unsigned short *tlist = reinterpret_cast<unsigned short *>(tkey + SLOTS);
-    unsigned int *lhist = reinterpret_cast<unsigned int *>(tlist + SLOTS);
+    unsigned int *lhist = reinterpret_cast<unsigned int *>(tlist + (kBT / 64) * kRegion);
 
     __shared__ long long s_seg;
-    __shared__ int s_occ;
     __shared__ int s_range[2];
     // the exact path's constants (read only by that rare path)
     __shared__ ExactParams s_ex;
@@ -396,19 +418,22 @@ void batch_kernel(BatchArgs a)
     // waves contending for one hot bin's LDS word)
     // (the wave index is uniform: the offset stays in a scalar register)
     unsigned int *const lhist_w = lhist + uniform_i32(((tid >> 6) % a.hist_copies) * a.hist_lds_words);
-    if (tid == 0) s_occ = 0;
     __syncthreads();
     // upper bound on the counts added to any LDS bin since the last spill
     long long binned = 0;
-    // entries this wave listed since the last flush, and the half of s_wcnt
-    // the next check uses (both uniform)
+    // entries this wave listed since the last flush (its part of the list),
+    // and the half of s_wcnt the next check uses (both uniform)
     int my_cnt = 0;
     int par = 0;
+    // every wave's list count at the last check (uniform)
+    int wcnt[kBT / 64] = {};
+    // this wave's part of the list (a uniform offset)
+    unsigned short *const tlist_w = tlist + (tid >> 6) * kRegion;
     // the current slab's key box (uniform)
     SlabBox box{0u, 0u, 0u, 1u, 1u, 1u, 1u};
 
-    // Appends the slots that lanes newly occupied (slot >= 0) to the list:
-    // one LDS atomic per wave reserves the positions.
+    // Appends the slots that lanes newly occupied (slot >= 0) to this wave's
+    // part of the list (positions from its own count: no atomics).
     // new_entries: the slots hold new entries (not survivors of a flush)
     auto append = [&](const int *slot, auto count_tag, bool new_entries) {
         constexpr int kN = decltype(count_tag)::value;
@@ -420,18 +445,17 @@ void batch_kernel(BatchArgs a)
             cnt += __popcll(m[j]);
         }
         if (cnt == 0) return;
+        int base = my_cnt;
         my_cnt += cnt;
         if (new_entries) n_created += (unsigned long long)cnt;
-        int base = 0;
-        if (lane == 0) base = atomicAdd(&s_occ, cnt);
-        // lane 0 is active here (every lane calls append): a scalar read
-        // of its value, not a cross-lane LDS permute
-        base = __builtin_amdgcn_readfirstlane(base);
+        if (my_cnt > kRegion) {   // cannot happen (the bound above); reported, never written past
+            if (lane == 0) atomicOr(&a.ctl->error, 2);
+        }
 #pragma unroll
         for (int j = 0; j < kN; ++j) {
             const int below = __builtin_amdgcn_mbcnt_hi((unsigned)(m[j] >> 32),
                                                         __builtin_amdgcn_mbcnt_lo((unsigned)m[j], 0u));
-            if (slot[j] >= 0 && base + below < SLOTS) tlist[base + below] = (unsigned short)slot[j];
+            if (slot[j] >= 0 && base + below < kRegion) tlist_w[base + below] = (unsigned short)slot[j];
             base += __popcll(m[j]);
         }
     };
@@ -441,17 +465,31 @@ void batch_kernel(BatchArgs a)
     // re-inserted into the cleared table.  Threads walk the dense list of
     // used slots, so the work follows the entries, not the table size, and
     // the bin edges of two entries are loaded before either is compared.
-    // after_check: called right after an occupancy check's barrier, which
-    // already ordered every wave's appends before this read of s_occ.
+    // after_check: called right after an occupancy check, whose barrier
+    // already ordered every wave's appends and whose counts are in wcnt.
     auto flush = [&](unsigned bound, bool after_check) {
         // raised issue priority while the workgroup's waves are tied together
         // by the flush barriers (measured: about 1 % on cfg 4)
         __builtin_amdgcn_s_setprio(2);
-        if (!after_check) __syncthreads();
-        int n = uniform_i32(s_occ);
+        if (!after_check) {
+            // the counts as an occupancy check publishes them
+            if (lane == 0) s_wcnt[par][tid >> 6] = my_cnt;
+            __syncthreads();
+#pragma unroll
+            for (int w = 0; w < kBT / 64; ++w) wcnt[w] = s_wcnt[par][w];
+            par ^= 1;
+        }
+        // list position idx in [0, n) lies in wave w's part at idx - start[w]
+        int start[kBT / 64];
+        int n = 0;
+#pragma unroll
+        for (int w = 0; w < kBT / 64; ++w) {
+            start[w] = n;
+            n += min(wcnt[w], kRegion);
+        }
         if (n > kLP * kBT) {   // cannot happen (the bound above); reported, never read past the list
             if (tid == 0) atomicOr(&a.ctl->error, 2);
-            n = kLP * kBT < SLOTS ? kLP * kBT : SLOTS;
+            n = kLP * kBT;
         }
         unsigned kk[kLP];
         unsigned long long k0[kLP], k1[kLP];
@@ -471,7 +509,13 @@ void batch_kernel(BatchArgs a)
                 if (r >= kLP) continue;
                 const int idx = tid + r * kBT;
                 if (idx < n) {
-                    const int sl = min((int)tlist[idx], SLOTS - 1);
+                    int part = 0;
+#pragma unroll
+                    for (int w = 1; w < kBT / 64; ++w) part += (idx >= start[w]) ? 1 : 0;
+                    int first = start[0];
+#pragma unroll
+                    for (int w = 1; w < kBT / 64; ++w) first = (part == w) ? start[w] : first;
+                    const int sl = min((int)tlist[part * kRegion + (idx - first)], SLOTS - 1);
                     // read and clear in one exchange each (the slot is this
                     // thread's alone until the barrier)
                     const unsigned k = atomicExch(&tkey[sl], kEmpty32);
@@ -530,7 +574,6 @@ void batch_kernel(BatchArgs a)
             }
         }
         __syncthreads();
-        if (tid == 0) s_occ = 0;
         my_cnt = 0;
         __syncthreads();
         // entries walked by this wave's lanes (rows of 64 list positions
@@ -843,13 +886,16 @@ void batch_kernel(BatchArgs a)
                     // after one barrier: a wave that runs on and appends more writes
                     // only its count for the next check, into the other half, so the
                     // counts read here cannot change under a slower wave (which would
-                    // otherwise flush alone).  The sum equals s_occ at the barrier.
+                    // otherwise flush alone).
                     if (lane == 0) s_wcnt[par][tid >> 6] = my_cnt;
                     if (tid == kBT - 1) s_layer[par] = last_layer;
                     __syncthreads();
                     int occ = 0;
 #pragma unroll
-                    for (int w = 0; w < kBT / 64; ++w) occ += s_wcnt[par][w];
+                    for (int w = 0; w < kBT / 64; ++w) {
+                        wcnt[w] = s_wcnt[par][w];
+                        occ += wcnt[w];
+                    }
                     const int lb = uniform_i32(s_layer[par]);
                     par ^= 1;
                     if (occ >= kFlushAt) {
