@@ -392,6 +392,9 @@ void batch_kernel(BatchArgs a)
     __shared__ int s_range[2];
     // the exact path's constants (read only by that rare path)
     __shared__ ExactParams s_ex;
+    // the histogram descriptors, read by the flushes from LDS (they then
+    // hold no scalar registers through the point loop)
+    __shared__ HistDev s_hist[kMaxHists];
     // entries each wave listed since the last flush, and a lower bound on the
     // layer of every later point, published at each occupancy check into
     // alternating halves
@@ -414,6 +417,7 @@ void batch_kernel(BatchArgs a)
         if (kWords == 2) tacc1[s] = 0;
     }
     for (int w = tid; w < a.hist_lds_words * a.hist_copies; w += kBT) lhist[w] = 0;
+    if (tid < kMaxHists) s_hist[tid] = a.hist[tid];
     // this wave's copy of the privatised histograms (more copies, fewer
     // waves contending for one hot bin's LDS word)
     // (the wave index is uniform: the offset stays in a scalar register)
@@ -546,7 +550,7 @@ void batch_kernel(BatchArgs a)
 #pragma unroll
             for (int h = 0; h < kMaxHists; ++h) {
                 if (h >= a.n_hists) break;
-                const HistDev &hd = a.hist[h];
+                const HistDev &hd = s_hist[h];
                 int bb[2];
                 double lo[2], hi[2];
 #pragma unroll
