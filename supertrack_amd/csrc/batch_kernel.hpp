@@ -577,23 +577,25 @@ void batch_kernel(BatchArgs a)
                 }
             }
         }
+        // every entry is read and cleared before any is re-inserted
         __syncthreads();
         my_cnt = 0;
-        __syncthreads();
         // entries walked by this wave's lanes (rows of 64 list positions
         // from wave_first, n uniform: scalar arithmetic)
         unsigned long long walked = 0;
 #pragma unroll
         for (int r = 0; r < kLP; ++r) walked += (unsigned long long)min(max(n - (r * kBT + wave_first), 0), 64);
-        // the few incomplete entries (a z-window of spheres) are re-inserted
-        // into the cleared table; most waves have none and skip the step
+        // The few incomplete entries (a z-window of spheres) are re-inserted
+        // into the cleared table; most waves have none and skip the step.
+        // Waves that have none go back to their points at once: a new point
+        // may insert a survivor's key first, and the survivor then adds its
+        // sums to that entry (listed by the point's wave).
         bool keep = false;
 #pragma unroll
         for (int r = 0; r < kLP; ++r) keep |= kk[r] != kEmpty32;
         if (__ballot(keep) != 0ull) {
             // first probes of every row at once (independent exchanges, one
-            // wait), then the rare collisions slot by slot; keys are unique,
-            // so a probe ends at the first empty slot
+            // wait), then the rare collisions slot by slot
             int placed[kLP];
             unsigned prev[kLP];
 #pragma unroll
@@ -610,13 +612,18 @@ void batch_kernel(BatchArgs a)
                 if (kk[r] == kEmpty32) continue;
                 unsigned int h = (unsigned)placed[r];
 #pragma unroll 1
-                for (int probe = 1; probe < SLOTS && prev[r] != kEmpty32; ++probe) {
+                for (int probe = 1; probe < SLOTS && prev[r] != kEmpty32 && prev[r] != kk[r]; ++probe) {
                     h = next_slot<SLOTS>(h);
                     prev[r] = atomicCAS(&tkey[h], kEmpty32, kk[r]);
                 }
-                placed[r] = (int)h;
-                atomicAdd(&tacc0[h], k0[r]);
-                if (kWords == 2) atomicAdd(&tacc1[h], k1[r]);
+                // listed again only where it made a new entry
+                placed[r] = prev[r] == kEmpty32 ? (int)h : -1;
+                // (a full table, excluded by its size, would drop the sums,
+                // which the conservation check reports)
+                if (prev[r] == kEmpty32 || prev[r] == kk[r]) {
+                    atomicAdd(&tacc0[h], k0[r]);
+                    if (kWords == 2) atomicAdd(&tacc1[h], k1[r]);
+                }
             }
             unsigned long long kept = 0;
 #pragma unroll
@@ -625,7 +632,6 @@ void batch_kernel(BatchArgs a)
             append(placed, std::integral_constant<int, kLP>(), false);
         }
         n_retired += walked;
-        __syncthreads();
         __builtin_amdgcn_s_setprio(0);
     };
 
