@@ -7,7 +7,8 @@ smaller track counts through ``make_library(..., n_tracks=...)``.
   cfg1  1 track of ~4.9e3 points, FP64, 10 oversamples, lin 3000 bins
   cfg2  256 tracks of 2e3-1e4 points, 100 oversamples, plain FP64 path
   cfg3  4096 tracks, log-uniform 1e3-1e6 points, 100 oversamples, log 1000 bins, batched FP64
-  cfg4  cfg3's library in FP32, D = 2 nm (64-bit keys), 1000 oversamples, batched FP32
+  cfg4  cfg3's library in FP32, D = 2 nm (N = 2500: reference keys beyond 32 bits), 1000 oversamples,
+        batched FP32
   cfg5  16384 tracks, 1e3-2e6 points, three geometries by 1024-track chunk, 1000 oversamples,
         lin 3000 + log 1000 bins
 """
