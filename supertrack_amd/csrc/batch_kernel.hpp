@@ -773,7 +773,9 @@ void batch_kernel(BatchArgs a)
             if (multi) __syncthreads();
             if (b_lo < b_hi) {
                 const int64_t p_end = td.offset + min((int64_t)b_hi * kBlk, (int64_t)td.n);
-                const int64_t p_safe = td.offset + (int64_t)b_lo * kBlk;   // a point of the range
+                // point indices relative to the track's first point (td.n <=
+                // 2^22: 32-bit lane offsets from uniform base addresses)
+                const int r_end = (int)(p_end - td.offset);
                 binned += (long long)(b_hi - b_lo) * kBlk;
                 // Software pipeline: the points of the next iteration are loaded
                 // before the current ones are scored.  Lanes past the end of the
@@ -783,10 +785,18 @@ void batch_kernel(BatchArgs a)
                 auto fetch = [&](int b, PointT *dst) {
 #pragma unroll
                     for (int j = 0; j < kPPT; ++j) {
-                        int64_t i = td.offset + (int64_t)(b + j) * kBlk + tid;
-                        if (!(b + j < b_hi && i < p_end)) i = p_safe;
-                        if constexpr (MODE == ST_F64) dst[j] = make_double4(a.x64[i], a.y64[i], a.z64[i], a.e64[i]);
-                        else dst[j] = a.p32[i];
+                        // past the range's last point: that point again
+                        const unsigned r = (b + j < b_hi) ? min((unsigned)((b + j) * kBlk + tid), (unsigned)r_end - 1u)
+                                                          : (unsigned)(b_lo * kBlk);
+                        if constexpr (MODE == ST_F64) {
+                            const unsigned o8 = r * 8u;
+                            dst[j] = make_double4(*(const double *)((const char *)(a.x64 + td.offset) + o8),
+                                                  *(const double *)((const char *)(a.y64 + td.offset) + o8),
+                                                  *(const double *)((const char *)(a.z64 + td.offset) + o8),
+                                                  *(const double *)((const char *)(a.e64 + td.offset) + o8));
+                        } else {
+                            dst[j] = *(const float4 *)((const char *)(a.p32 + td.offset) + r * 16u);
+                        }
                     }
                 };
                 fetch(b_lo, nxt);
@@ -807,8 +817,8 @@ void batch_kernel(BatchArgs a)
 #pragma unroll
                     for (int j = 0; j < kPPT; ++j) {
                         if (b + j >= b_hi) break;
-                        const int64_t i = c + (int64_t)j * kBlk + tid;
-                        const bool valid = i < p_end;
+                        const int ri = (b + j) * kBlk + tid;   // relative to the track's first point
+                        const bool valid = ri < r_end;
                         bool in = false;
                         unsigned key = 0;
                         long long v0 = 0, v1 = 0;
@@ -857,7 +867,8 @@ void batch_kernel(BatchArgs a)
                             if constexpr (MODE == ST_F64)
                                 in = cand && exact_local(cur[j].x, cur[j].y, cur[j].z, key);
                             else
-                                in = cand && exact_local(a.x64[i], a.y64[i], a.z64[i], key);
+                                in = cand && exact_local(a.x64[td.offset + ri], a.y64[td.offset + ri],
+                                                         a.z64[td.offset + ri], key);
                         }
                         if constexpr (MODE == ST_F64) {
                             if (in) Acc<ST_F64>::split(cur[j].w / g.denom, v0, v1);
