@@ -276,7 +276,10 @@ int st_batch_stats(st_batch *b, int64_t *stats3);
  * into the sphere tables, [1] those of the entries retired from them (binned
  * or outside every histogram), [2] table entries created, [3] entries
  * retired.  A pass that loses no entry has [0] == [1] and [2] == [3];
- * st_batch_sync returns ST_ESTATE otherwise.  No counterpart in the
+ * st_batch_sync returns ST_ESTATE otherwise.  [0] and [1] are the same in
+ * every pass over the same work; [2] and [3] may vary by a few between
+ * passes (an entry kept across a flush can merge into one that a new point
+ * made first: one more created and retired).  No counterpart in the
  * reference: a check on the fused kernel. */
 int st_batch_balance(st_batch *b, uint64_t *balance4);
 /* Test hook: record (segment, key, sum) of every scored sphere of the next

@@ -79,10 +79,12 @@ def test_static_pass_equals_batch_run_twice(precision):
             assert stats[0] == wstats["points_scored"]
             assert stats[1] == wstats["fp64_rechecks"]
             assert stats[2] == wstats["points_in_sphere"]
-            # conservation totals, and equal to the object-based run's
+            # conservation totals; the energy words equal the object-based
+            # run's (the entry counts may differ by flush survivors that merged
+            # into an entry a new point made first, depending on timing)
             u = stats.view(np.uint64)
             assert u[3] == u[4] and u[5] == u[6] and u[5] > 0
-            assert (int(u[3]), int(u[5])) == (wbal["energy_in"], wbal["entries_created"])
+            assert int(u[3]) == wbal["energy_in"]
     finally:
         big.free()
         small.free()
