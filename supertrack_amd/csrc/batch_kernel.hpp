@@ -471,7 +471,7 @@ void batch_kernel(BatchArgs a)
     // the bin edges of two entries are loaded before either is compared.
     // after_check: called right after an occupancy check, whose barrier
     // already ordered every wave's appends and whose counts are in wcnt.
-    auto flush = [&](unsigned bound, bool after_check) {
+    auto flush = [&](unsigned bound, bool after_check) __attribute__((always_inline)) {
         // raised issue priority while the workgroup's waves are tied together
         // by the flush barriers (measured: about 1 % on cfg 4)
         __builtin_amdgcn_s_setprio(2);
@@ -799,13 +799,14 @@ void batch_kernel(BatchArgs a)
                         }
                     }
                 };
+                PointT buf1[kPPT];
                 fetch(b_lo, nxt);
-                int since = 0;
-                for (int b = b_lo; b < b_hi; b += kPPT) {
-                    PointT cur[kPPT];
-#pragma unroll
-                    for (int j = 0; j < kPPT; ++j) cur[j] = nxt[j];
-                    if (b + kPPT < b_hi) fetch(b + kPPT, nxt);
+                // One iteration: rows b .. b + kPPT - 1, their points in nxt
+                // (buffer 0) or buf1 (buffer 1); then, if it is a checking
+                // iteration (a compile-time choice: the check and its flush are
+                // inlined once per loop), the occupancy check.
+                auto iteration = [&](const int b, auto buffer, auto checking) __attribute__((always_inline)) {
+                    PointT(&cur)[kPPT] = decltype(buffer)::value ? buf1 : nxt;
                     const int64_t c = td.offset + (int64_t)b * kBlk;
                     // the iteration's last point (thread kBT - 1, last row):
                     // its local layer, or kLayerUnsure if its rounding was
@@ -900,8 +901,7 @@ void batch_kernel(BatchArgs a)
                     }
                     append(fresh, std::integral_constant<int, kPPT>(), true);
                     const int64_t next = c + (int64_t)kPPT * kBlk;
-                    if (++since < kCheck || next >= p_end) continue;
-                    since = 0;
+                    if (!decltype(checking)::value || next >= p_end) return;
                     // Every wave must take the same decision.  Each publishes its own
                     // count into this check's half of s_wcnt and sums all of them
                     // after one barrier: a wave that runs on and appends more writes
@@ -937,6 +937,28 @@ void batch_kernel(BatchArgs a)
                         if (lb_exact >= (int)box.nz) bound = kEmpty32;
                         else if (lb_exact > 0) bound = (unsigned)lb_exact * box.nxy;
                         flush(bound, true);
+                    }
+                };
+                using Buf0 = std::integral_constant<int, 0>;
+                using Buf1 = std::integral_constant<int, 1>;
+                if constexpr (kCheck == 2) {
+                    // two iterations per trip, their points loaded into
+                    // alternating buffers (no register copies), the check after
+                    // the second (every second iteration from the range's start)
+                    for (int b = b_lo; b < b_hi; b += 2 * kPPT) {
+                        if (b + kPPT < b_hi) fetch(b + kPPT, buf1);
+                        iteration(b, Buf0(), std::false_type());
+                        if (b + kPPT >= b_hi) break;
+                        if (b + 2 * kPPT < b_hi) fetch(b + 2 * kPPT, nxt);
+                        iteration(b + kPPT, Buf1(), std::true_type());
+                    }
+                } else {
+                    static_assert(kCheck == 1, "occupancy checks every one or two iterations");
+                    for (int b = b_lo; b < b_hi; b += kPPT) {
+#pragma unroll
+                        for (int j = 0; j < kPPT; ++j) buf1[j] = nxt[j];
+                        if (b + kPPT < b_hi) fetch(b + kPPT, nxt);
+                        iteration(b, Buf1(), std::true_type());
                     }
                 }
                 flush(kEmpty32, false);
