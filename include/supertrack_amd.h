@@ -236,9 +236,12 @@ typedef struct {
     int32_t use_graph;            /* capture the launch sequence once, replay */
     int32_t autotune;             /* pick launch parameters by timing          */
     int32_t block_threads;        /* 0 = default / autotuned                   */
-    int32_t table_slots;          /* LDS hash slots per workgroup: 1280 or 1920
+    int32_t table_slots;          /* LDS hash slots per workgroup: 1280 or 1472
                                      (one point per thread), 1536, 2048 or
-                                     4096; 0 = 1536, or 1280 where that fits
+                                     4096; 0 (with blocks_per_cu and
+                                     points_per_thread 0) = 1472 at six
+                                     workgroups per CU (ST_F32) where they
+                                     fit, else 1536, or 1280 where that fits
                                      more workgroups                          */
     int32_t blocks_per_cu;        /* persistent workgroups per CU; 0 = 5 (ST_F32),
                                      3 (ST_F64), capped by occupancy           */

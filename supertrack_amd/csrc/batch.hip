@@ -51,10 +51,10 @@ const void *kernel_ptr()
 template <int MODE>
 const void *pick_kernel_mode(int slots, int ppt)
 {
-    // 1280 slots hold the largest survivor set plus one point per thread only;
-    // 1920 slots are built for one point per thread only
+    // 1280 and 1472 slots hold the largest survivor set plus the points
+    // between two occupancy checks at one point per thread only
     if (slots == 1280) return ppt == 2 ? nullptr : kernel_ptr<MODE, 1280, 1>();
-    if (slots == 1920) return ppt == 2 ? nullptr : kernel_ptr<MODE, 1920, 1>();
+    if (slots == 1472) return ppt == 2 ? nullptr : kernel_ptr<MODE, 1472, 1>();
     if (slots == 1536) return ppt == 2 ? kernel_ptr<MODE, 1536, 2>() : kernel_ptr<MODE, 1536, 1>();
     if (slots == 2048) return ppt == 2 ? kernel_ptr<MODE, 2048, 2>() : kernel_ptr<MODE, 2048, 1>();
     return ppt == 2 ? kernel_ptr<MODE, 4096, 2>() : kernel_ptr<MODE, 4096, 1>();
@@ -273,12 +273,22 @@ int default_launch(int precision, int hist_words, const st_batch_opts &o, Launch
 {
     const bool f32 = precision == ST_F32;
     const int slots = o.table_slots ? o.table_slots : 1536;
-    if (slots != 1280 && slots != 1536 && slots != 1920 && slots != 2048 && slots != 4096)
-        return set_error(ST_EINVAL, "batched run: table_slots must be 1280, 1536, 1920, 2048 or 4096");
+    if (slots != 1280 && slots != 1472 && slots != 1536 && slots != 2048 && slots != 4096)
+        return set_error(ST_EINVAL, "batched run: table_slots must be 1280, 1472, 1536, 2048 or 4096");
     const int bpc = o.blocks_per_cu ? o.blocks_per_cu : (f32 ? 5 : 3);
     const int ppt = o.points_per_thread ? o.points_per_thread : (f32 ? 1 : 2);
     if (ppt != 1 && ppt != 2) return set_error(ST_EINVAL, "batched run: points_per_thread must be 1 or 2");
     const int copies = o.hist_copies ? o.hist_copies : 1;
+    // FP32, shape not given: 1472-slot tables at six workgroups per CU where
+    // they fit (cfg 4: 277 ms against 288 ms at 1536 x 5 x 1)
+    if (f32 && !o.table_slots && !o.blocks_per_cu && !o.points_per_thread) {
+        Launch six;
+        if (configure_launch(precision, hist_words, 1472, 6, 1, copies, &six) == ST_OK && six.blocks_per_cu == 6) {
+            *out = six;
+            return ST_OK;
+        }
+        clear_error();
+    }
     int rc = configure_launch(precision, hist_words, slots, bpc, ppt, copies, out);
     if (rc) return rc;
     // FP32 with fewer than five workgroups per CU (large privatised
@@ -370,7 +380,7 @@ int autotune(st_batch *b)
                                   hipMemcpyDeviceToHost, b->stream));
     }
     ST_HIP_TRY(hipStreamSynchronize(b->stream));
-    const int cand_slots[] = {1280, 1536, 1920, 2048, 4096};
+    const int cand_slots[] = {1280, 1472, 1536, 2048, 4096};
     const int cand_bpc[] = {2, 3, 4, 5, 6, 8};
     const int cand_copies[] = {1, 2, 4};
     float best = 1e30f;
