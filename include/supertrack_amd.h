@@ -239,12 +239,14 @@ typedef struct {
     int32_t table_slots;          /* LDS hash slots per workgroup: 1280 or 1472
                                      (one point per thread), 1536, 2048 or
                                      4096; 0 (with blocks_per_cu and
-                                     points_per_thread 0) = 1472 at six
-                                     workgroups per CU (ST_F32) where they
-                                     fit, else 1536, or 1280 where that fits
-                                     more workgroups                          */
-    int32_t blocks_per_cu;        /* persistent workgroups per CU; 0 = 5 (ST_F32),
-                                     3 (ST_F64), capped by occupancy           */
+                                     points_per_thread 0) = 1280 at seven or
+                                     1472 at six workgroups per CU (ST_F32)
+                                     where they fit, else 1536, or 1280
+                                     where that fits more workgroups          */
+    int32_t blocks_per_cu;        /* persistent workgroups per CU; 0 = 7 or 6
+                                     (ST_F32, with the tables above), else 5
+                                     (ST_F32), 3 (ST_F64), capped by
+                                     occupancy                                 */
     int32_t points_per_thread;    /* 1 or 2 per iteration; 0 = 1 (ST_F32 at five
                                      workgroups per CU, else 2), 2 (ST_F64)    */
     int32_t hist_copies;          /* LDS copies of the histograms per workgroup

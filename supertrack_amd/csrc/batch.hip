@@ -279,15 +279,20 @@ int default_launch(int precision, int hist_words, const st_batch_opts &o, Launch
     const int ppt = o.points_per_thread ? o.points_per_thread : (f32 ? 1 : 2);
     if (ppt != 1 && ppt != 2) return set_error(ST_EINVAL, "batched run: points_per_thread must be 1 or 2");
     const int copies = o.hist_copies ? o.hist_copies : 1;
-    // FP32, shape not given: 1472-slot tables at six workgroups per CU where
-    // they fit (cfg 4: 277 ms against 288 ms at 1536 x 5 x 1)
+    // FP32, shape not given: 1280-slot tables at seven workgroups per CU
+    // where they fit, else 1472 slots at six (cfg 4 on one box: 273.8 ms,
+    // 278.0 ms, against 288 ms at 1536 x 5 x 1)
     if (f32 && !o.table_slots && !o.blocks_per_cu && !o.points_per_thread) {
-        Launch six;
-        if (configure_launch(precision, hist_words, 1472, 6, 1, copies, &six) == ST_OK && six.blocks_per_cu == 6) {
-            *out = six;
-            return ST_OK;
+        const int shapes[2][2] = {{1280, 7}, {1472, 6}};
+        for (const auto &sh : shapes) {
+            Launch occ;
+            if (configure_launch(precision, hist_words, sh[0], sh[1], 1, copies, &occ) == ST_OK &&
+                occ.blocks_per_cu == sh[1]) {
+                *out = occ;
+                return ST_OK;
+            }
+            clear_error();
         }
-        clear_error();
     }
     int rc = configure_launch(precision, hist_words, slots, bpc, ppt, copies, out);
     if (rc) return rc;

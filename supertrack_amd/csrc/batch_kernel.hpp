@@ -348,16 +348,16 @@ constexpr int list_region(int slots, int ppt)
 // the persistent fused kernel
 // ---------------------------------------------------------------------------
 // kPPT: library blocks (points per thread) loaded together per iteration.
-// The FP32 kernel with tables of 1280 or 1536 slots is asked to fit five
-// workgroups per CU (20 waves instead of 16 hide more of the LDS-atomic and
-// load latency), with 1472 slots six (with 1000 histogram bins, 26 KB of LDS
-// per workgroup: 24 waves per CU, measured faster despite the registers
-// spilled at 80 per lane).
+// Waves per SIMD the FP32 one-point-per-thread kernels are compiled for:
+// more waves hide more of the LDS-atomic and load latency, and every step up
+// measured faster despite the registers spilled (outside the point path):
+// 1536 slots five (96 registers per lane), 1472 slots six (80; 26 KB of LDS
+// per workgroup with 1000 histogram bins), 1280 slots seven (72; 22.7 KB).
 template <int MODE, int SLOTS, int kPPT>
 constexpr int waves_per_eu()
 {
     if (MODE != ST_F32 || kPPT != 1) return 1;
-    return SLOTS == 1472 ? 6 : (SLOTS == 1280 || SLOTS == 1536) ? 5 : 1;
+    return SLOTS == 1280 ? 7 : SLOTS == 1472 ? 6 : SLOTS == 1536 ? 5 : 1;
 }
 
 template <int MODE, int SLOTS, int kPPT>
