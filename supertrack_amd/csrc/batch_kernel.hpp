@@ -322,17 +322,19 @@ __device__ int wave_lower_bound(int n, Pred pred)
     return lo;
 }
 
-// Occupancy checks happen every kCheck iterations; a flush starts at a
-// quarter-full table.
+// Iterations between occupancy checks: two where the table holds the
+// largest survivor set plus two iterations' points at one point per thread
+// (1472 and 1536 slots), else one.
 constexpr int check_every(int slots, int ppt)
 {
     return ((slots == 1472 || slots == 1536) && ppt == 1) ? 2 : 1;
 }
 
 // Capacity of each wave's part of the slot list.  A wave lists at most the
-// entries it walked in the last flush (kLP rows of 64: 64 * kLP) or, at an
-// occupancy check that did not flush, fewer than the flush threshold; then
-// at most the points of the iterations until the next check.
+// survivors it re-inserted in the last flush (no more than the entries it
+// walked: kLP rows of 64) or, at an occupancy check that did not flush, fewer
+// than the flush threshold; then at most the points of the iterations until
+// the next check.
 constexpr int list_region(int slots, int ppt)
 {
     const int flush_at = slots / 4;
@@ -365,17 +367,17 @@ __global__ __launch_bounds__(kBT) __attribute__((amdgpu_waves_per_eu(waves_per_e
 void batch_kernel(BatchArgs a)
 {
     static_assert(SLOTS == 1280 || SLOTS == 1472 || SLOTS == 1536 || SLOTS == 2048 || SLOTS == 4096, "table sizes");
-    // occupancy never exceeds max(half table, survivors) + one iteration's points
     // A flush starts once the table is a quarter full: a low load factor
     // keeps the linear-probe chains (dependent LDS round trips, paid by the
     // whole wave for its longest lane) short; a quarter was fastest on MI355X
     // (an eighth flushes too often, a half probes too long).
     constexpr int kFlushAt = SLOTS / 4;
-    // occupancy never exceeds max(flush threshold, survivors) + one iteration's points
     // iterations between occupancy checks (each check is a workgroup barrier)
     constexpr int kCheck = check_every(SLOTS, kPPT);
+    // occupancy never exceeds max(flush threshold, survivors) + the points of
+    // the iterations between two checks, kept under 90 % of the table
     static_assert((kFlushAt > kMaxWindow ? kFlushAt : kMaxWindow) + kCheck * kPPT * kBlk <= SLOTS * 9 / 10,
-                  "the table must hold the survivors of a flush plus an iteration");
+                  "the table must hold the survivors of a flush plus the points between two checks");
     // entries at a flush: those kept by the previous flush (at most kMaxWindow)
     // or below the threshold, plus the points of the iterations since the
     // last check
@@ -408,7 +410,9 @@ void batch_kernel(BatchArgs a)
     const int lane = tid & 63;
     const int wave_first = uniform_i32(tid & ~63);   // first thread of this wave
     // per-thread statistics, summed at exit
-    unsigned int n_scored = 0, n_flagged = 0, n_in = 0;
+    unsigned int n_flagged = 0, n_in = 0;
+    // points scored by the workgroup (uniform): the lengths of its ranges
+    unsigned long long n_scored = 0;
     // conservation: energy words into and out of the table, entries created
     // and retired (a lost entry breaks one of the two equalities)
     unsigned long long bal_in = 0, bal_out = 0;
@@ -780,11 +784,12 @@ void batch_kernel(BatchArgs a)
                 // 2^22: 32-bit lane offsets from uniform base addresses)
                 const int r_end = (int)(p_end - td.offset);
                 binned += (long long)(b_hi - b_lo) * kBlk;
+                n_scored += (unsigned long long)(r_end - b_lo * kBlk);
                 // Software pipeline: the points of the next iteration are loaded
                 // before the current ones are scored.  Lanes past the end of the
                 // range load a valid point and are masked off.
                 using PointT = typename std::conditional<MODE == ST_F64, double4, float4>::type;
-                PointT nxt[kPPT];
+                PointT buf0[kPPT];
                 auto fetch = [&](int b, PointT *dst) {
 #pragma unroll
                     for (int j = 0; j < kPPT; ++j) {
@@ -803,13 +808,13 @@ void batch_kernel(BatchArgs a)
                     }
                 };
                 PointT buf1[kPPT];
-                fetch(b_lo, nxt);
-                // One iteration: rows b .. b + kPPT - 1, their points in nxt
+                fetch(b_lo, buf0);
+                // One iteration: rows b .. b + kPPT - 1, their points in buf0
                 // (buffer 0) or buf1 (buffer 1); then, if it is a checking
                 // iteration (a compile-time choice: the check and its flush are
                 // inlined once per loop), the occupancy check.
                 auto iteration = [&](const int b, auto buffer, auto checking) __attribute__((always_inline)) {
-                    PointT(&cur)[kPPT] = decltype(buffer)::value ? buf1 : nxt;
+                    PointT(&cur)[kPPT] = decltype(buffer)::value ? buf1 : buf0;
                     const int64_t c = td.offset + (int64_t)b * kBlk;
                     // the iteration's last point (thread kBT - 1, last row):
                     // its local layer, or kLayerUnsure if its rounding was
@@ -879,7 +884,6 @@ void batch_kernel(BatchArgs a)
                         } else {
                             v0 = rint_to_i64((double)cur[j].w * g.yscale);
                         }
-                        n_scored += valid;
                         n_in += in;
                         if (in) {
                             // linear probing; the loop is kept rolled: nearly every
@@ -952,15 +956,17 @@ void batch_kernel(BatchArgs a)
                         if (b + kPPT < b_hi) fetch(b + kPPT, buf1);
                         iteration(b, Buf0(), std::false_type());
                         if (b + kPPT >= b_hi) break;
-                        if (b + 2 * kPPT < b_hi) fetch(b + 2 * kPPT, nxt);
+                        if (b + 2 * kPPT < b_hi) fetch(b + 2 * kPPT, buf0);
                         iteration(b + kPPT, Buf1(), std::true_type());
                     }
                 } else {
                     static_assert(kCheck == 1, "occupancy checks every one or two iterations");
+                    // one iteration per trip: the current points copied to
+                    // buffer 1, the next ones loaded into buffer 0
                     for (int b = b_lo; b < b_hi; b += kPPT) {
 #pragma unroll
-                        for (int j = 0; j < kPPT; ++j) buf1[j] = nxt[j];
-                        if (b + kPPT < b_hi) fetch(b + kPPT, nxt);
+                        for (int j = 0; j < kPPT; ++j) buf1[j] = buf0[j];
+                        if (b + kPPT < b_hi) fetch(b + kPPT, buf0);
                         iteration(b, Buf1(), std::true_type());
                     }
                 }
@@ -991,16 +997,17 @@ void batch_kernel(BatchArgs a)
     }
     {
         // per-lane counters summed over the wave; n_created / n_retired are
-        // already per wave
-        unsigned long long c[5] = {n_scored, n_flagged, n_in, bal_in, bal_out};
+        // already per wave, n_scored per workgroup
+        unsigned long long c[4] = {n_flagged, n_in, bal_in, bal_out};
 #pragma unroll
         for (int d = 32; d >= 1; d >>= 1) {
 #pragma unroll
-            for (int k = 0; k < 5; ++k) c[k] += __shfl_xor(c[k], d, 64);
+            for (int k = 0; k < 4; ++k) c[k] += __shfl_xor(c[k], d, 64);
         }
         if (lane == 0) {
+            if (tid == 0) atomicAdd(&a.ctl->stats[0], n_scored);
 #pragma unroll
-            for (int k = 0; k < 5; ++k) atomicAdd(&a.ctl->stats[k], c[k]);
+            for (int k = 0; k < 4; ++k) atomicAdd(&a.ctl->stats[k + 1], c[k]);
             atomicAdd(&a.ctl->stats[5], n_created);
             atomicAdd(&a.ctl->stats[6], n_retired);
         }
