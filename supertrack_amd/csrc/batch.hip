@@ -386,7 +386,7 @@ int autotune(st_batch *b)
     }
     ST_HIP_TRY(hipStreamSynchronize(b->stream));
     const int cand_slots[] = {1280, 1472, 1536, 2048, 4096};
-    const int cand_bpc[] = {2, 3, 4, 5, 6, 8};
+    const int cand_bpc[] = {2, 3, 4, 5, 6, 7, 8};
     const int cand_copies[] = {1, 2, 4};
     float best = 1e30f;
     int best_s = b->ln.slots, best_b = b->ln.blocks_per_cu, best_p = b->ln.ppt, best_c = b->ln.hist_copies;
