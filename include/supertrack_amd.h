@@ -245,7 +245,7 @@ typedef struct {
                                      where that fits more workgroups          */
     int32_t blocks_per_cu;        /* persistent workgroups per CU; 0 = 7 or 6
                                      (ST_F32, with the tables above), else 5
-                                     (ST_F32), 3 (ST_F64), capped by
+                                     (ST_F32), 4 (ST_F64), capped by
                                      occupancy                                 */
     int32_t points_per_thread;    /* 1 or 2 per iteration; 0 = 1 (ST_F32 at five
                                      workgroups per CU, else 2), 2 (ST_F64)    */

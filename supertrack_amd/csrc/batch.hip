@@ -275,7 +275,7 @@ int default_launch(int precision, int hist_words, const st_batch_opts &o, Launch
     const int slots = o.table_slots ? o.table_slots : 1536;
     if (slots != 1280 && slots != 1472 && slots != 1536 && slots != 2048 && slots != 4096)
         return set_error(ST_EINVAL, "batched run: table_slots must be 1280, 1472, 1536, 2048 or 4096");
-    const int bpc = o.blocks_per_cu ? o.blocks_per_cu : (f32 ? 5 : 3);
+    const int bpc = o.blocks_per_cu ? o.blocks_per_cu : (f32 ? 5 : 4);
     const int ppt = o.points_per_thread ? o.points_per_thread : (f32 ? 1 : 2);
     if (ppt != 1 && ppt != 2) return set_error(ST_EINVAL, "batched run: points_per_thread must be 1 or 2");
     const int copies = o.hist_copies ? o.hist_copies : 1;

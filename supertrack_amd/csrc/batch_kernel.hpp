@@ -350,15 +350,18 @@ constexpr int list_region(int slots, int ppt)
 // the persistent fused kernel
 // ---------------------------------------------------------------------------
 // kPPT: library blocks (points per thread) loaded together per iteration.
-// Waves per SIMD the FP32 one-point-per-thread kernels are compiled for:
-// more waves hide more of the LDS-atomic and load latency, and every step up
-// measured faster despite the registers spilled (outside the point path):
-// 1536 slots five (96 registers per lane), 1472 slots six (80; 26 KB of LDS
-// per workgroup with 1000 histogram bins), 1280 slots seven (72; 22.7 KB).
+// Waves per SIMD the kernels are compiled for: more waves hide more of the
+// LDS-atomic and load latency, and every step up measured faster despite the
+// registers spilled (outside the point path).  ST_F32 at one point per
+// thread: 1536 slots five (96 registers per lane), 1472 slots six (80; 26 KB
+// of LDS per workgroup with 1000 histogram bins), 1280 slots seven (72;
+// 22.7 KB).  ST_F64 with 1536 slots at two points per thread: four (128, no
+// spills; 39 KB of LDS).
 template <int MODE, int SLOTS, int kPPT>
 constexpr int waves_per_eu()
 {
-    if (MODE != ST_F32 || kPPT != 1) return 1;
+    if (MODE == ST_F64) return (SLOTS == 1536 && kPPT == 2) ? 4 : 1;
+    if (kPPT != 1) return 1;
     return SLOTS == 1280 ? 7 : SLOTS == 1472 ? 6 : SLOTS == 1536 ? 5 : 1;
 }
 
