@@ -297,10 +297,11 @@ int default_launch(int precision, int hist_words, const st_batch_opts &o, Launch
     int rc = configure_launch(precision, hist_words, slots, bpc, ppt, copies, out);
     if (rc) return rc;
     // FP32 with fewer than five workgroups per CU (large privatised
-    // histograms): a 1280-slot table if it fits more workgroups; else the
-    // table at four workgroups per CU if it fits them (the full cfg 5 run:
-    // 1536 x 4 x 1 in 1406 ms against 1564 ms at 1536 x 3 x 2); else two
-    // points per thread
+    // histograms): a 1280-slot table if it fits more workgroups; else two
+    // points per thread, at four workgroups per CU if the one-point table
+    // fits four (the full cfg 5 run on one box: 1536 x 4 x 2 in 1290 ms,
+    // 1536 x 4 x 1 in 1312 ms; earlier 1536 x 3 x 2, with a kernel that fit
+    // three, 1564 ms against 1406 ms at 1536 x 4 x 1)
     if (f32 && !o.points_per_thread && out->blocks_per_cu < 5) {
         const int fit = out->blocks_per_cu;
         if (!o.table_slots) {
@@ -312,8 +313,13 @@ int default_launch(int precision, int hist_words, const st_batch_opts &o, Launch
             }
             clear_error();
         }
-        if (fit >= 4) return ST_OK;
-        return configure_launch(precision, hist_words, slots, bpc, 2, copies, out);
+        if (fit < 4) return configure_launch(precision, hist_words, slots, bpc, 2, copies, out);
+        Launch two;
+        if (configure_launch(precision, hist_words, slots, bpc, 2, copies, &two) == ST_OK &&
+            two.blocks_per_cu >= fit)
+            *out = two;
+        clear_error();
+        return ST_OK;
     }
     return ST_OK;
 }
